@@ -1,0 +1,154 @@
+#!/usr/bin/env python3
+"""Benchmark of the XPBI substep on one MI355X.
+
+Workload: 1M-particle Drucker-Prager sand -- an 80x160x80 seeded-jitter column
+at r = 0.005 m (1,024,000 particles; E 3.5e5 Pa, nu 0.3, rho 2000, phi 30 deg),
+dt = 1/6000 s, 20 Jacobi iterations.  Before warm-up the column is advanced
+`--settle` substeps (untimed) so that it has landed and deforms; a free-falling
+column would skip the constraint sweeps.  A step is one substep of all
+particles; `value` = particles x substeps / wall seconds of the timed region,
+with every array resident in HBM.
+
+Also reported: the per-kernel HIP-event times of the same timed region, the
+HBM roofline of the dominant kernel (solveA), Jacobi iterations to residual
+1e-4, and the CPU oracle's substep on the host cores (a reported baseline).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [os.path.join(REPO, "xpbi-webgpu_amd")]
+
+METRIC = "particle-substeps/s @1M DP sand, Jacobi iters to 1e-4: learned vs plain"
+HBM_PEAK_GBS = 8000.0
+# compulsory bytes per particle per launch (every per-particle array read or
+# written once; neighbour re-reads not counted)
+ALG_BYTES = {"solveA": 164, "solveB": 96, "computeL": 68, "xsph": 48, "updateF": 148, "integrate": 64,
+             "reorder": 172, "pbfA": 36, "pbfB": 52}
+GATHER = ("computeL", "solveA", "solveB", "xsph", "updateF")
+
+
+def iterations_to_residual(sim, tol, cap, water):
+    """Plain Jacobi from lambda = 0: first iteration whose residual is <= tol
+    (residual = ||C + alpha~ lambda + gamma dt dC.v|| / ||C at iteration 0||)."""
+    import numpy as np
+    sim.set_residual_monitor(True)
+    sim.substep(cap, water)
+    r = sim.read_residuals()
+    sim.set_residual_monitor(False)
+    if r.shape[0] == 0 or r[0, 1] <= 0:
+        return None, []
+    rel = np.sqrt(r[:, 0] / r[0, 1])
+    hit = np.nonzero(rel <= tol)[0]
+    return (int(hit[0]) + 1 if hit.size else None), rel
+
+
+def cpu_baseline(sim, scene, iterations, threads):
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle_py
+    from xpbi import pack_materials
+    oracle_py.build()
+    st = oracle_py.State(sim.N, sim.numCells)
+    st.pos[:] = sim.read("POS")
+    st.velA[:] = sim.read("VEL")
+    st.F[:] = sim.read("F")
+    st.aux[:] = sim.read("AUX")
+    mats = pack_materials(scene["materials"], sim.V0)
+    t0 = time.perf_counter()
+    oracle_py.substep(st, sim.params, mats, iterations, scene["hasWater"], nthreads=threads)
+    dt = time.perf_counter() - t0
+    return dict(value=sim.N / dt, unit="particle-substeps/s", cores=threads, kind="port",
+                sample=f"1 substep ({iterations} Jacobi iterations) of the settled {sim.N}-particle column, "
+                       f"f32 C++ oracle, OpenMP {threads} threads, {dt:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--iterations", type=int, default=20)
+    ap.add_argument("--settle", type=int, default=300)
+    ap.add_argument("--residual-cap", type=int, default=200)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-residual", action="store_true")
+    ap.add_argument("--config", default="headline", choices=["headline", "c2"])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the substep engine has no multi-GPU path in this build "
+              "(each GPU box has one MI355X); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import xpbi
+    from xpbi import scenes
+    scene = scenes.config_headline() if args.config == "headline" else scenes.config_c2()
+    I = args.iterations
+    sim = xpbi.Simulation(scene, device=0)
+    sim.writeParams(xpbi.params_from_scene(scene))
+    N = sim.N
+    print(f"[bench] {N} particles, {sim.numCells} cells; settling {args.settle} substeps", file=sys.stderr,
+          flush=True)
+    sim.step(args.settle, I, False)
+    sim.step(args.warmup, I, False)
+    sim.sync()
+    print("[bench] timed region", file=sys.stderr, flush=True)
+    total_ms, per = sim.time_substeps_detail(args.steps, I, False)
+    ms_per_step = total_ms / args.steps
+    value = N * args.steps / (total_ms / 1e3)
+
+    kern = {}
+    for name, (ms, cnt) in per.items():
+        avg = ms / max(cnt, 1)
+        ent = dict(avg_ms=round(avg, 5), launches=cnt, share=round(ms / total_ms, 4))
+        if name in ALG_BYTES:
+            ent["alg_GBs"] = round(ALG_BYTES[name] * N / (avg / 1e3) / 1e9, 1)
+        kern[name] = ent
+    a_ms = per["solveA"][0] / per["solveA"][1]
+    achieved = ALG_BYTES["solveA"] * N / (a_ms / 1e3) / 1e9
+    traffic = None
+    tf = os.path.join(REPO, "profiles", "solveA_headline_traffic.txt")
+    if os.path.exists(tf):
+        for line in open(tf):
+            if line.startswith("traffic_bytes_per_launch="):
+                traffic = float(line.split("=")[1])
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel="solveA",
+                    alg_bytes_per_launch=ALG_BYTES["solveA"] * N)
+    g_ms = sum(per[k][0] for k in GATHER if k in per)
+    g_bytes = (264 + 260 * I) * N * args.steps
+    gather = dict(achieved_GBs=round(g_bytes / (g_ms / 1e3) / 1e9, 1),
+                  frac=round(g_bytes / (g_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4))
+
+    resid = None
+    if not args.no_residual:
+        print("[bench] iterations to residual 1e-4", file=sys.stderr, flush=True)
+        it_plain, rel = iterations_to_residual(sim, 1e-4, args.residual_cap, False)
+        resid = dict(tol=1e-4, plain_iterations=it_plain, cap=args.residual_cap,
+                     residual_at_20=(float(rel[19]) if len(rel) > 19 else None), learned_iterations=None)
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        print(f"[bench] CPU oracle baseline on {threads} threads", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(sim, scene, I, threads)
+
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "particle-substeps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"DP sand column 80x160x80 r=0.005 ({N} particles), plain Jacobi"
+                   if args.config == "headline" else f"DP sand column 40x80x40 r=0.01 ({N} particles)",
+                   "particles": N, "iterations": I, "dt": scene["dt"], "settle_substeps": args.settle},
+        "roofline": roofline, "gather_kernels_roofline": gather, "kernels": kern,
+        "jacobi_to_residual": resid, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    sim.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,198 @@
+/*
+ * xpbi.h -- C ABI of the MI355X XPBI substep engine (libxpbi.so).
+ *
+ * The reference drives its substep from JavaScript through WebGPU:
+ *   class Simulation            web/js/sim.js:9-273
+ *   constructor(device, scene)  web/js/sim.js:10-30 (grid set-up)
+ *   createBuffers()             web/js/sim.js:32-82 (state upload)
+ *   writeMaterials(materials)   web/js/sim.js:84-99 (64-B records)
+ *   writeParams(p)              web/js/sim.js:159-186 (256-B record)
+ *   encodeSubstep(enc, I, w)    web/js/sim.js:220-272 (one substep)
+ *   profileSubstep(I, w)        web/js/sim.js:199-215 (per-pass timing)
+ *   readPos / readVel hooks     web/js/main.js:291-312
+ * Each entry point below replaces one of those; the Node N-API addon and the
+ * Python ctypes binding wrap them 1:1.  All functions return XPBI_OK (0) or a
+ * negative XPBI_ERR_* code; xpbi_last_error() gives the message.
+ *
+ * Threading: a sim owns one HIP stream; calls on one sim are not reentrant.
+ * xpbi_substep / xpbi_step enqueue asynchronously; xpbi_read synchronises.
+ * After every substep the current velocity is the "velA" of the reference
+ * (web/js/sim.js:217, :268-271).
+ */
+#ifndef XPBI_H
+#define XPBI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XPBI_OK 0
+#define XPBI_ERR_ARG (-1)     /* bad argument (null, size mismatch, range) */
+#define XPBI_ERR_HIP (-2)     /* HIP runtime error */
+#define XPBI_ERR_STATE (-3)   /* call out of order (e.g. substep before params) */
+#define XPBI_ERR_NOMEM (-4)   /* device allocation failed */
+
+/* Material kinds, web/js/sim.js:5-7 and web/js/shaders/common.js:13-19 */
+enum {
+  XPBI_MAT_ELASTIC = 0,
+  XPBI_MAT_DP = 1,
+  XPBI_MAT_NACC = 2,
+  XPBI_MAT_VM = 3,
+  XPBI_MAT_HB = 4,
+  XPBI_MAT_SNOW = 5,
+  XPBI_MAT_WATER = 6
+};
+
+/* 64-byte material record: struct Material, web/js/shaders/common.js:21-33,
+ * packed as web/js/sim.js:84-99 does (mu, lambda from E, nu on the host). */
+typedef struct xpbi_material {
+  float mu, lam, density;
+  uint32_t kind;
+  float p0, p1, p2, p3; /* DP(alpha,coh) NACC(alpha0,beta,xi,M) VM(sigY) HB(sigY,h,eta) SNOW(thC,thS,xi) */
+  float color[4];
+  float boundFriction;
+  float mass; /* density * V0 */
+  float pad0, pad1;
+} xpbi_material;
+
+/* 256-byte parameter record: struct Params, web/js/shaders/common.js:35-60,
+ * word offsets as written by web/js/sim.js:159-186. */
+typedef struct xpbi_params {
+  float gridOrigin[3];
+  float cellSize;
+  int32_t gridDims[3];
+  uint32_t numCells;
+  float domainMin[3];
+  float dt;
+  float domainMax[3];
+  float h;
+  float gravity[3];
+  float support;
+  float colliderPos[3];
+  float colliderR;
+  float colliderVel[3];
+  float colliderActive;
+  uint32_t N;
+  float V0, xsphC, omega, pbfEps, sCorrK, time, contactOmega, kDamp;
+  float pad[27];
+} xpbi_params;
+
+/* Scene description: the scene object of web/js/scenes.js:94-176 as consumed
+ * by the Simulation constructor and createBuffers (web/js/sim.js:10-82). */
+typedef struct xpbi_scene_desc {
+  uint32_t N;
+  const float* positions;  /* 3N */
+  const float* tags;       /* N: pos.w = matIdx + colour fraction < 0.49 (sim.js:63); NULL -> matIds */
+  const uint32_t* matIds;  /* N, used when tags == NULL */
+  const float* states;     /* N plastic state (aux.z), NULL -> 0 */
+  const float* velocities; /* 3N, NULL -> 0 */
+  double h;                /* particle spacing / kernel radius; support = 2h */
+  double domainMin[3];
+  double domainMax[3];
+  int device; /* HIP device ordinal */
+} xpbi_scene_desc;
+
+/* Fields for xpbi_read / xpbi_write, in the reference's buffer layouts
+ * (web/js/sim.js:32-50, web/js/shaders/simkernels.js:18-57). */
+enum {
+  XPBI_FIELD_POS = 0,       /* 16N: vec4 (x, y, z, tag) */
+  XPBI_FIELD_VEL = 1,       /* 16N: vec4 (v, w = V^n after scatter) -- velA */
+  XPBI_FIELD_F = 2,         /* 48N: 3 columns, each a vec4 with pad 0 */
+  XPBI_FIELD_L = 3,         /* 48N: same layout, read only */
+  XPBI_FIELD_AUX = 4,       /* 16N: (V^n, lambda, plastic state, pbf lambda) */
+  XPBI_FIELD_SP = 5,        /* 48N: columns of dlam*G*L, selfDv in the w lanes; read only */
+  XPBI_FIELD_CELLSTART = 6, /* 4*numCells: exclusive start offsets (cellTab after scatter); read only */
+  XPBI_FIELD_PERM = 7,      /* 4N: sortedIdx of the last sort; read only */
+  XPBI_FIELD_ID = 8,        /* 4N: persistent particle id, permuted with the particles */
+  XPBI_FIELD_COUNT = 9
+};
+
+/* Stages for xpbi_substep_partial (staged parity checks); XPBI_STAGE_FULL is a
+ * whole substep.  Order follows web/js/sim.js:246-266. */
+enum {
+  XPBI_STAGE_SORT = 1,
+  XPBI_STAGE_REORDER = 2,
+  XPBI_STAGE_COMPUTE_L = 3,
+  XPBI_STAGE_SOLVE = 4,
+  XPBI_STAGE_XSPH = 5,
+  XPBI_STAGE_UPDATE_F = 6,
+  XPBI_STAGE_FULL = 7
+};
+
+typedef struct xpbi_sim xpbi_sim;
+
+/* new Simulation(device, scene) + createBuffers(): web/js/sim.js:10-82.
+ * Grid: cs = 2h, origin = domainMin - 2cs, dims = ceil((max-min)/cs) + 4
+ * (sim.js:20-26), computed in double like the JavaScript host. F = I,
+ * aux = (h^3, 0, state, 0), id = 0..N-1. Materials must be written next. */
+int xpbi_create(const xpbi_scene_desc* desc, xpbi_sim** out);
+
+/* Frees every device buffer (the reference has no destroy; sim.js:9). */
+void xpbi_destroy(xpbi_sim* sim);
+
+/* Message of the last failed call on sim (sim == NULL: last create failure). */
+const char* xpbi_last_error(const xpbi_sim* sim);
+
+/* Grid as computed at create (the JavaScript host packs it into Params the
+ * way web/js/sim.js:174-176 does). */
+int xpbi_grid_info(const xpbi_sim* sim, float origin[3], float* cell_size, int32_t dims[3], uint32_t* num_cells,
+                   uint32_t* n_particles);
+
+/* writeMaterials(materials): web/js/sim.js:84-99; n <= 8 records. */
+int xpbi_write_materials(xpbi_sim* sim, const xpbi_material* mats, int n);
+
+/* writeParams(p): web/js/sim.js:159-186.  The grid words and N must equal the
+ * sim's own (xpbi_grid_info); support must not exceed cellSize. */
+int xpbi_write_params(xpbi_sim* sim, const xpbi_params* p);
+
+/* encodeSubstep(enc, iterations, hasWater): web/js/sim.js:220-272. */
+int xpbi_substep(xpbi_sim* sim, int iterations, int has_water);
+
+/* The frame loop's substep chunk (web/js/main.js:406-413): substeps x substep. */
+int xpbi_step(xpbi_sim* sim, int substeps, int iterations, int has_water);
+
+/* A substep that stops after `stage` (XPBI_STAGE_*), for staged parity. */
+int xpbi_substep_partial(xpbi_sim* sim, int iterations, int has_water, int stage);
+
+/* readPos/readVel (web/js/main.js:291-312) generalised to every buffer;
+ * bytes must equal the field's size.  Synchronises the sim's stream. */
+int xpbi_read(xpbi_sim* sim, int field, void* dst, size_t bytes);
+
+/* Overwrite a state field (POS, VEL, F, AUX, ID) between substeps. */
+int xpbi_write(xpbi_sim* sim, int field, const void* src, size_t bytes);
+
+/* Wait for all enqueued work of the sim. */
+int xpbi_sync(xpbi_sim* sim);
+
+/* profileSubstep(iterations, hasWater): web/js/sim.js:199-215.  Runs one
+ * substep with a HIP event pair around every kernel launch; writes up to cap
+ * per-launch milliseconds to ms and their count to *count; labels via
+ * xpbi_profile_label.  Repeated launches (solve iterations) are listed each. */
+int xpbi_profile_substep(xpbi_sim* sim, int iterations, int has_water, float* ms, int cap, int* count);
+const char* xpbi_profile_label(const xpbi_sim* sim, int k);
+
+/* Enqueue `substeps` substeps between two HIP events on the sim's stream and
+ * return the elapsed milliseconds (inputs stay resident in HBM). */
+int xpbi_time_substeps(xpbi_sim* sim, int substeps, int iterations, int has_water, float* ms);
+
+/* As xpbi_time_substeps, with a HIP event after every kernel launch inside the
+ * timed region; per distinct label (xpbi_profile_label(sim, k), k < *nlabels)
+ * label_ms[k] is the summed milliseconds and label_count[k] the launches. */
+int xpbi_time_substeps_detail(xpbi_sim* sim, int substeps, int iterations, int has_water, float* total_ms,
+                              float* label_ms, int* label_count, int cap, int* nlabels);
+
+/* Residual monitor (no reference counterpart: the reference runs a fixed
+ * iteration count, web/js/sim.js:256).  When enabled, every solveA reduces
+ * r_k = ||C + alpha~ lambda + gamma dt dC.v||_2 over solid particles; after a
+ * substep, out[2k] = sum of squares at iteration k, out[2k+1] = sum C^2 at k. */
+int xpbi_set_residual_monitor(xpbi_sim* sim, int enabled);
+int xpbi_read_residuals(xpbi_sim* sim, double* out, int cap, int* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XPBI_H */

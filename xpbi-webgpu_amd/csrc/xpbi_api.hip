@@ -1,0 +1,557 @@
+// C ABI of libxpbi (declared in include/xpbi.h).  Host orchestration of the
+// substep: grid set-up and state upload (web/js/sim.js:10-82), material and
+// parameter records (sim.js:84-99, :159-186), the dispatch schedule
+// (sim.js:220-272), per-launch profiling (sim.js:188-215) and buffer readback
+// (web/js/main.js:289-313).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "xpbi_internal.hpp"
+
+using namespace xpbi;
+
+namespace {
+
+thread_local std::string g_create_err;
+
+int fail(Sim* s, int code, const std::string& msg) {
+  if (s) s->err = msg;
+  else g_create_err = msg;
+  return code;
+}
+int hip_fail(Sim* s, hipError_t e, const char* what) {
+  return fail(s, XPBI_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T> hipError_t dalloc(Sim* s, T** p, size_t count) {
+  size_t bytes = count * sizeof(T);
+  if (bytes == 0) bytes = 16;
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) return e;
+  s->allocs.push_back(q);
+  *p = (T*)q;
+  return hipSuccess;
+}
+
+void free_all(Sim* s) {
+  if (s->stream) hipStreamSynchronize(s->stream);
+  for (void* p : s->allocs) hipFree(p);
+  s->allocs.clear();
+  for (hipEvent_t e : s->events) hipEventDestroy(e);
+  s->events.clear();
+  if (s->stream) hipStreamDestroy(s->stream);
+  s->stream = nullptr;
+}
+
+// derived constants, formed in f32 with no contraction so that they equal the
+// per-pair expressions of common.js evaluated in f32
+#pragma clang fp contract(off)
+void build_kp(Sim* s) {
+  const xpbi_params& p = s->params;
+  KP k{};
+  k.ox = p.gridOrigin[0]; k.oy = p.gridOrigin[1]; k.oz = p.gridOrigin[2]; k.cell = p.cellSize;
+  k.gx = p.gridDims[0]; k.gy = p.gridDims[1]; k.gz = p.gridDims[2]; k.C = p.numCells;
+  k.dminx = p.domainMin[0]; k.dminy = p.domainMin[1]; k.dminz = p.domainMin[2]; k.dt = p.dt;
+  k.dmaxx = p.domainMax[0]; k.dmaxy = p.domainMax[1]; k.dmaxz = p.domainMax[2]; k.h = p.h;
+  k.grx = p.gravity[0]; k.gry = p.gravity[1]; k.grz = p.gravity[2]; k.support = p.support;
+  k.cpx = p.colliderPos[0]; k.cpy = p.colliderPos[1]; k.cpz = p.colliderPos[2]; k.cR = p.colliderR;
+  k.cvx = p.colliderVel[0]; k.cvy = p.colliderVel[1]; k.cvz = p.colliderVel[2]; k.cActive = p.colliderActive;
+  k.N = p.N; k.V0 = p.V0; k.xsphC = p.xsphC; k.omega = p.omega;
+  k.pbfEps = p.pbfEps; k.sCorrK = p.sCorrK; k.contactOmega = p.contactOmega; k.kDamp = p.kDamp;
+  const float h = p.h, H = p.support;
+  k.supp2 = H * H;
+  k.sigW = 21.0f / ((float)(16.0 * 3.14159265) * h * h * h);
+  k.poly6C = 315.0f / ((float)(64.0 * 3.14159265) * std::pow(H, 9.0f));
+  k.spikyC = -45.0f / (3.14159265f * std::pow(H, 6.0f));
+  {
+    float r = 0.2f * H;
+    float x = H * H - r * r;
+    k.w0 = (r >= H) ? 0.0f : k.poly6C * x * x * x;
+    float x0 = H * H - 0.0f * 0.0f;
+    k.poly6_0 = k.poly6C * x0 * x0 * x0;
+  }
+  k.alphaT = (1.0f / p.V0) / (p.dt * p.dt);
+  k.vmax = 0.45f * H / p.dt;
+  k.dvCap = 0.2f * H / p.dt;
+  k.minD = 0.75f * h;
+  k.gxy = (u32)(p.gridDims[0] * p.gridDims[1]);
+  s->kp = k;
+}
+#pragma clang fp contract(on)
+
+int check_launch(Sim* s, const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(s, e, what);
+  return XPBI_OK;
+}
+
+int run_substep(Sim* s, int iterations, int has_water, int stop) {
+  if (!s->haveParams) return fail(s, XPBI_ERR_STATE, "substep before xpbi_write_params");
+  if (!s->haveMats) return fail(s, XPBI_ERR_STATE, "substep before xpbi_write_materials");
+  if (iterations < 0) return fail(s, XPBI_ERR_ARG, "iterations < 0");
+  if (stop < XPBI_STAGE_SORT || stop > XPBI_STAGE_FULL) return fail(s, XPBI_ERR_ARG, "bad stage");
+  hipSetDevice(s->dev);
+  if (s->N == 0) return XPBI_OK;
+  launch_sort(s);
+  s->sorted = true;
+  if (stop == XPBI_STAGE_SORT) return check_launch(s, "sort");
+  launch_reorder(s);
+  if (stop == XPBI_STAGE_REORDER) return check_launch(s, "reorder");
+  launch_solve_stages(s, iterations, has_water != 0, stop);
+  return check_launch(s, "substep");
+}
+
+size_t field_bytes(const Sim* s, int field) {
+  switch (field) {
+    case XPBI_FIELD_POS:
+    case XPBI_FIELD_VEL:
+    case XPBI_FIELD_AUX: return 16ull * s->N;
+    case XPBI_FIELD_F:
+    case XPBI_FIELD_L:
+    case XPBI_FIELD_SP: return 48ull * s->N;
+    case XPBI_FIELD_CELLSTART: return 4ull * s->C;
+    case XPBI_FIELD_PERM:
+    case XPBI_FIELD_ID: return 4ull * s->N;
+    default: return 0;
+  }
+}
+
+// planes (a, b, c) <-> reference 3-column vec4 layout (pad 0, or w from `w4`)
+void planes_to_ref(u32 N, const float* a, const float* b, const float* c, const float* w4, float* out) {
+  for (u32 i = 0; i < N; i++) {
+    const float* A = a + 4ull * i;
+    const float* B = b + 4ull * i;
+    float m[9] = {A[0], A[1], A[2], A[3], B[0], B[1], B[2], B[3], c[i]};
+    float* o = out + 12ull * i;
+    for (int col = 0; col < 3; col++) {
+      o[4 * col + 0] = m[3 * col + 0];
+      o[4 * col + 1] = m[3 * col + 1];
+      o[4 * col + 2] = m[3 * col + 2];
+      o[4 * col + 3] = w4 ? w4[4ull * i + col] : 0.0f;
+    }
+  }
+}
+void ref_to_planes(u32 N, const float* in, float* a, float* b, float* c) {
+  for (u32 i = 0; i < N; i++) {
+    const float* o = in + 12ull * i;
+    float m[9];
+    for (int col = 0; col < 3; col++)
+      for (int r = 0; r < 3; r++) m[3 * col + r] = o[4 * col + r];
+    for (int k = 0; k < 4; k++) { a[4ull * i + k] = m[k]; b[4ull * i + k] = m[4 + k]; }
+    c[i] = m[8];
+  }
+}
+
+int d2h(Sim* s, void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return XPBI_OK;
+  hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+  return e == hipSuccess ? XPBI_OK : hip_fail(s, e, "hipMemcpy D2H");
+}
+int h2d(Sim* s, void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return XPBI_OK;
+  hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+  return e == hipSuccess ? XPBI_OK : hip_fail(s, e, "hipMemcpy H2D");
+}
+
+}  // namespace
+
+extern "C" {
+
+int xpbi_create(const xpbi_scene_desc* d, xpbi_sim** out) {
+  if (!out) return fail(nullptr, XPBI_ERR_ARG, "out is null");
+  *out = nullptr;
+  if (!d) return fail(nullptr, XPBI_ERR_ARG, "desc is null");
+  if (d->N > 0 && !d->positions) return fail(nullptr, XPBI_ERR_ARG, "positions is null");
+  if (d->N > 0 && !d->tags && !d->matIds) return fail(nullptr, XPBI_ERR_ARG, "tags and matIds are both null");
+  if (!(d->h > 0.0) || !std::isfinite(d->h)) return fail(nullptr, XPBI_ERR_ARG, "h must be > 0");
+  // grid: sim.js:20-26 (double arithmetic, as in JavaScript)
+  const double cs = 2.0 * d->h;
+  const int pad = 2;
+  double dimsD[3];
+  for (int k = 0; k < 3; k++) {
+    if (!(d->domainMax[k] >= d->domainMin[k])) return fail(nullptr, XPBI_ERR_ARG, "domainMax < domainMin");
+    dimsD[k] = std::ceil((d->domainMax[k] - d->domainMin[k]) / cs) + 2 * pad;
+  }
+  double cells = dimsD[0] * dimsD[1] * dimsD[2];
+  if (cells >= 4294967295.0 || dimsD[0] * dimsD[1] >= 2147483647.0)
+    return fail(nullptr, XPBI_ERR_ARG, "grid too large");
+
+  xpbi_sim* h = new xpbi_sim();
+  Sim* s = &h->s;
+  s->dev = d->device;
+  s->N = d->N;
+  s->h = d->h;
+  for (int k = 0; k < 3; k++) {
+    s->dims[k] = (int)dimsD[k];
+    s->origin[k] = (float)(d->domainMin[k] - pad * cs);
+  }
+  s->cell = (float)cs;
+  s->C = (u32)cells;
+
+  hipError_t e = hipSetDevice(s->dev);
+  if (e != hipSuccess) { int r = hip_fail(nullptr, e, "hipSetDevice"); delete h; return r; }
+  e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { int r = hip_fail(nullptr, e, "hipStreamCreate"); delete h; return r; }
+
+  const u32 N = s->N, C = s->C;
+  const u32 gN = (N + 255) / 256;
+  bool ok = true;
+  auto A = [&](hipError_t r) { if (r != hipSuccess) ok = false; };
+  for (int k = 0; k < 2; k++) {
+    A(dalloc(s, &s->set[k].pos, N));
+    A(dalloc(s, &s->set[k].Fa, N));
+    A(dalloc(s, &s->set[k].Fb, N));
+    A(dalloc(s, &s->set[k].Fc, N));
+    A(dalloc(s, &s->set[k].aux, N));
+    A(dalloc(s, &s->set[k].id, N));
+    A(dalloc(s, &s->vel[k], N));
+    A(dalloc(s, &s->keys[k], N));
+    A(dalloc(s, &s->vals[k], N));
+  }
+  A(dalloc(s, &s->L.a, N));
+  A(dalloc(s, &s->L.b, N));
+  A(dalloc(s, &s->L.c, N));
+  A(dalloc(s, &s->Sp.a, N));
+  A(dalloc(s, &s->Sp.b, N));
+  A(dalloc(s, &s->Sp.c, N));
+  A(dalloc(s, &s->selfDv, N));
+  A(dalloc(s, &s->cellStart, (size_t)C + 1));
+  A(dalloc(s, &s->count, C));
+  size_t np = scan_partials_needed(C);
+  size_t nr = radix_hist_needed(N);
+  size_t np2 = scan_partials_needed(nr);
+  A(dalloc(s, &s->scanPart, np > np2 ? np : np2));
+  A(dalloc(s, &s->radixHist, nr > 0 ? nr : 1));
+  A(dalloc(s, &s->dMats, 8));
+  A(dalloc(s, &s->residPart, 2ull * (gN > 0 ? gN : 1)));
+  A(dalloc(s, &s->residOut, 2ull * kMaxResidIters));
+  if (!ok) {
+    free_all(s);
+    delete h;
+    return fail(nullptr, XPBI_ERR_NOMEM, "device allocation failed");
+  }
+  s->perm = s->vals[0];
+  s->sortedKeys = s->keys[0];
+
+  // initial state: sim.js:52-81
+  const float V0 = (float)(d->h * d->h * d->h);
+  std::vector<float> pos(4ull * N), vel(4ull * N, 0.0f), aux(4ull * N, 0.0f), Fa(4ull * N, 0.0f),
+      Fb(4ull * N, 0.0f), Fc(N, 1.0f);
+  std::vector<u32> id(N), zeros((size_t)C + 1, 0u);
+  for (u32 i = 0; i < N; i++) {
+    pos[4ull * i + 0] = d->positions[3ull * i + 0];
+    pos[4ull * i + 1] = d->positions[3ull * i + 1];
+    pos[4ull * i + 2] = d->positions[3ull * i + 2];
+    pos[4ull * i + 3] = d->tags ? d->tags[i] : (float)d->matIds[i];
+    if (d->velocities) {
+      vel[4ull * i + 0] = d->velocities[3ull * i + 0];
+      vel[4ull * i + 1] = d->velocities[3ull * i + 1];
+      vel[4ull * i + 2] = d->velocities[3ull * i + 2];
+    }
+    aux[4ull * i + 0] = V0;
+    aux[4ull * i + 2] = d->states ? d->states[i] : 0.0f;
+    Fa[4ull * i + 0] = 1.0f;  // c0.x
+    Fb[4ull * i + 0] = 1.0f;  // c1.y
+    id[i] = i;
+  }
+  int r = XPBI_OK;
+  if (N > 0) {
+    if (!r) r = h2d(s, s->set[0].pos, pos.data(), 16ull * N);
+    if (!r) r = h2d(s, s->vel[0], vel.data(), 16ull * N);
+    if (!r) r = h2d(s, s->set[0].aux, aux.data(), 16ull * N);
+    if (!r) r = h2d(s, s->set[0].Fa, Fa.data(), 16ull * N);
+    if (!r) r = h2d(s, s->set[0].Fb, Fb.data(), 16ull * N);
+    if (!r) r = h2d(s, s->set[0].Fc, Fc.data(), 4ull * N);
+    if (!r) r = h2d(s, s->set[0].id, id.data(), 4ull * N);
+    if (!r) r = h2d(s, s->vals[0], id.data(), 4ull * N);
+  }
+  if (!r) r = h2d(s, s->cellStart, zeros.data(), 4ull * ((size_t)C + 1));
+  if (r) {
+    std::string m = s->err;
+    free_all(s);
+    delete h;
+    return fail(nullptr, r, m);
+  }
+  s->events.resize(kMaxProfileMarks);
+  for (auto& ev : s->events) hipEventCreate(&ev);
+  s->labels.resize(kMaxProfileMarks);
+  *out = h;
+  return XPBI_OK;
+}
+
+void xpbi_destroy(xpbi_sim* h) {
+  if (!h) return;
+  hipSetDevice(h->s.dev);
+  free_all(&h->s);
+  delete h;
+}
+
+const char* xpbi_last_error(const xpbi_sim* h) { return h ? h->s.err.c_str() : g_create_err.c_str(); }
+
+int xpbi_grid_info(const xpbi_sim* h, float origin[3], float* cell_size, int32_t dims[3], uint32_t* num_cells,
+                   uint32_t* n_particles) {
+  if (!h) return XPBI_ERR_ARG;
+  const Sim& s = h->s;
+  for (int k = 0; k < 3; k++) {
+    if (origin) origin[k] = s.origin[k];
+    if (dims) dims[k] = s.dims[k];
+  }
+  if (cell_size) *cell_size = s.cell;
+  if (num_cells) *num_cells = s.C;
+  if (n_particles) *n_particles = s.N;
+  return XPBI_OK;
+}
+
+int xpbi_write_materials(xpbi_sim* h, const xpbi_material* mats, int n) {
+  if (!h) return XPBI_ERR_ARG;
+  Sim* s = &h->s;
+  if (n < 0 || n > 8) return fail(s, XPBI_ERR_ARG, "material count must be 0..8");
+  if (n > 0 && !mats) return fail(s, XPBI_ERR_ARG, "mats is null");
+  xpbi_material m[8];
+  std::memset(m, 0, sizeof(m));
+  for (int k = 0; k < n; k++) m[k] = mats[k];
+  hipSetDevice(s->dev);
+  hipError_t e = hipStreamSynchronize(s->stream);
+  if (e != hipSuccess) return hip_fail(s, e, "hipStreamSynchronize");
+  int r = h2d(s, s->dMats, m, sizeof(m));
+  if (r) return r;
+  std::memcpy(s->mats, m, sizeof(m));
+  s->haveMats = true;
+  return XPBI_OK;
+}
+
+int xpbi_write_params(xpbi_sim* h, const xpbi_params* p) {
+  if (!h) return XPBI_ERR_ARG;
+  Sim* s = &h->s;
+  if (!p) return fail(s, XPBI_ERR_ARG, "params is null");
+  for (int k = 0; k < 3; k++) {
+    if (p->gridOrigin[k] != s->origin[k] || p->gridDims[k] != s->dims[k])
+      return fail(s, XPBI_ERR_ARG, "params grid does not match the simulation grid");
+  }
+  if (p->cellSize != s->cell || p->numCells != s->C) return fail(s, XPBI_ERR_ARG, "params grid does not match");
+  if (p->N != s->N) return fail(s, XPBI_ERR_ARG, "params N does not match the particle count");
+  if (!(p->dt > 0.0f) || !(p->h > 0.0f)) return fail(s, XPBI_ERR_ARG, "dt and h must be > 0");
+  if (!(p->support > 0.0f) || p->support > p->cellSize)
+    return fail(s, XPBI_ERR_ARG, "support must be in (0, cellSize]");
+  s->params = *p;
+  build_kp(s);
+  s->haveParams = true;
+  return XPBI_OK;
+}
+
+int xpbi_substep(xpbi_sim* h, int iterations, int has_water) {
+  if (!h) return XPBI_ERR_ARG;
+  return run_substep(&h->s, iterations, has_water, XPBI_STAGE_FULL);
+}
+
+int xpbi_substep_partial(xpbi_sim* h, int iterations, int has_water, int stage) {
+  if (!h) return XPBI_ERR_ARG;
+  return run_substep(&h->s, iterations, has_water, stage);
+}
+
+int xpbi_step(xpbi_sim* h, int substeps, int iterations, int has_water) {
+  if (!h) return XPBI_ERR_ARG;
+  if (substeps < 0) return fail(&h->s, XPBI_ERR_ARG, "substeps < 0");
+  for (int k = 0; k < substeps; k++) {
+    int r = run_substep(&h->s, iterations, has_water, XPBI_STAGE_FULL);
+    if (r) return r;
+  }
+  return XPBI_OK;
+}
+
+int xpbi_sync(xpbi_sim* h) {
+  if (!h) return XPBI_ERR_ARG;
+  Sim* s = &h->s;
+  hipSetDevice(s->dev);
+  hipError_t e = hipStreamSynchronize(s->stream);
+  return e == hipSuccess ? XPBI_OK : hip_fail(s, e, "hipStreamSynchronize");
+}
+
+int xpbi_read(xpbi_sim* h, int field, void* dst, size_t bytes) {
+  if (!h) return XPBI_ERR_ARG;
+  Sim* s = &h->s;
+  size_t need = field_bytes(s, field);
+  if (field < 0 || field >= XPBI_FIELD_COUNT) return fail(s, XPBI_ERR_ARG, "unknown field");
+  if (bytes != need) return fail(s, XPBI_ERR_ARG, "byte count does not match the field size");
+  if (need > 0 && !dst) return fail(s, XPBI_ERR_ARG, "dst is null");
+  int r = xpbi_sync(h);
+  if (r) return r;
+  const u32 N = s->N;
+  PSet& S = s->set[s->cur];
+  switch (field) {
+    case XPBI_FIELD_POS: return d2h(s, dst, S.pos, need);
+    case XPBI_FIELD_VEL: return d2h(s, dst, s->vel[s->vcur], need);
+    case XPBI_FIELD_AUX: return d2h(s, dst, S.aux, need);
+    case XPBI_FIELD_CELLSTART: return d2h(s, dst, s->cellStart, need);
+    case XPBI_FIELD_PERM: return d2h(s, dst, s->perm, need);
+    case XPBI_FIELD_ID: return d2h(s, dst, S.id, need);
+    case XPBI_FIELD_F:
+    case XPBI_FIELD_L:
+    case XPBI_FIELD_SP: {
+      std::vector<float> a(4ull * N), b(4ull * N), c(N), w;
+      const float4 *pa, *pb;
+      const float* pc;
+      if (field == XPBI_FIELD_F) { pa = S.Fa; pb = S.Fb; pc = S.Fc; }
+      else if (field == XPBI_FIELD_L) { pa = s->L.a; pb = s->L.b; pc = s->L.c; }
+      else { pa = s->Sp.a; pb = s->Sp.b; pc = s->Sp.c; }
+      if ((r = d2h(s, a.data(), pa, 16ull * N))) return r;
+      if ((r = d2h(s, b.data(), pb, 16ull * N))) return r;
+      if ((r = d2h(s, c.data(), pc, 4ull * N))) return r;
+      if (field == XPBI_FIELD_SP) {
+        w.resize(4ull * N);
+        if ((r = d2h(s, w.data(), s->selfDv, 16ull * N))) return r;
+      }
+      planes_to_ref(N, a.data(), b.data(), c.data(), w.empty() ? nullptr : w.data(), (float*)dst);
+      return XPBI_OK;
+    }
+  }
+  return fail(s, XPBI_ERR_ARG, "unknown field");
+}
+
+int xpbi_write(xpbi_sim* h, int field, const void* src, size_t bytes) {
+  if (!h) return XPBI_ERR_ARG;
+  Sim* s = &h->s;
+  if (field < 0 || field >= XPBI_FIELD_COUNT) return fail(s, XPBI_ERR_ARG, "unknown field");
+  size_t need = field_bytes(s, field);
+  if (bytes != need) return fail(s, XPBI_ERR_ARG, "byte count does not match the field size");
+  if (need > 0 && !src) return fail(s, XPBI_ERR_ARG, "src is null");
+  int r = xpbi_sync(h);
+  if (r) return r;
+  const u32 N = s->N;
+  PSet& S = s->set[s->cur];
+  switch (field) {
+    case XPBI_FIELD_POS: return h2d(s, S.pos, src, need);
+    case XPBI_FIELD_VEL: return h2d(s, s->vel[s->vcur], src, need);
+    case XPBI_FIELD_AUX: return h2d(s, S.aux, src, need);
+    case XPBI_FIELD_ID: return h2d(s, S.id, src, need);
+    case XPBI_FIELD_F: {
+      std::vector<float> a(4ull * N), b(4ull * N), c(N);
+      ref_to_planes(N, (const float*)src, a.data(), b.data(), c.data());
+      if ((r = h2d(s, S.Fa, a.data(), 16ull * N))) return r;
+      if ((r = h2d(s, S.Fb, b.data(), 16ull * N))) return r;
+      return h2d(s, S.Fc, c.data(), 4ull * N);
+    }
+    default: return fail(s, XPBI_ERR_ARG, "field is read only");
+  }
+}
+
+int xpbi_profile_substep(xpbi_sim* h, int iterations, int has_water, float* ms, int cap, int* count) {
+  if (!h) return XPBI_ERR_ARG;
+  Sim* s = &h->s;
+  if (cap < 0 || (cap > 0 && !ms)) return fail(s, XPBI_ERR_ARG, "bad ms/cap");
+  int r = xpbi_sync(h);
+  if (r) return r;
+  s->profiling = true;
+  s->nmarks = 0;
+  hipEventRecord(s->events[0], s->stream);
+  r = run_substep(s, iterations, has_water, XPBI_STAGE_FULL);
+  s->profiling = false;
+  if (r) return r;
+  if ((r = xpbi_sync(h))) return r;
+  int n = s->nmarks;
+  for (int k = 0; k < n && k < cap; k++) {
+    float t = 0.0f;
+    hipEventElapsedTime(&t, s->events[k], s->events[k + 1]);
+    ms[k] = t;
+  }
+  if (count) *count = n;
+  return XPBI_OK;
+}
+
+const char* xpbi_profile_label(const xpbi_sim* h, int k) {
+  if (!h || k < 0 || k >= h->s.nmarks) return "";
+  return h->s.labels[k].c_str();
+}
+
+int xpbi_time_substeps(xpbi_sim* h, int substeps, int iterations, int has_water, float* ms) {
+  if (!h || !ms) return XPBI_ERR_ARG;
+  Sim* s = &h->s;
+  int r = xpbi_sync(h);
+  if (r) return r;
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipEventRecord(e0, s->stream);
+  for (int k = 0; k < substeps && !r; k++) r = run_substep(s, iterations, has_water, XPBI_STAGE_FULL);
+  hipEventRecord(e1, s->stream);
+  hipError_t e = hipEventSynchronize(e1);
+  float t = 0.0f;
+  hipEventElapsedTime(&t, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (r) return r;
+  if (e != hipSuccess) return hip_fail(s, e, "hipEventSynchronize");
+  *ms = t;
+  return XPBI_OK;
+}
+
+int xpbi_time_substeps_detail(xpbi_sim* h, int substeps, int iterations, int has_water, float* total_ms,
+                              float* label_ms, int* label_count, int cap, int* nlabels) {
+  if (!h || !total_ms || substeps < 0 || cap < 0 || (cap > 0 && (!label_ms || !label_count))) return XPBI_ERR_ARG;
+  Sim* s = &h->s;
+  int r = xpbi_sync(h);
+  if (r) return r;
+  // one event per launch: at most 16 launches per substep outside the solve loop
+  size_t need = (size_t)substeps * (size_t)(24 + 4 * (iterations > 0 ? iterations : 0)) + 2;
+  while (s->events.size() < need) {
+    hipEvent_t ev;
+    if (hipEventCreate(&ev) != hipSuccess) break;
+    s->events.push_back(ev);
+  }
+  if (s->labels.size() < s->events.size()) s->labels.resize(s->events.size());
+  s->profiling = true;
+  s->nmarks = 0;
+  hipEventRecord(s->events[0], s->stream);
+  for (int k = 0; k < substeps && !r; k++) r = run_substep(s, iterations, has_water, XPBI_STAGE_FULL);
+  s->profiling = false;
+  if (r) return r;
+  if ((r = xpbi_sync(h))) return r;
+  float tot = 0.0f;
+  hipEventElapsedTime(&tot, s->events[0], s->events[s->nmarks]);
+  *total_ms = tot;
+  std::vector<std::string> names;
+  std::vector<double> sums;
+  std::vector<int> counts;
+  for (int k = 0; k < s->nmarks; k++) {
+    float t = 0.0f;
+    hipEventElapsedTime(&t, s->events[k], s->events[k + 1]);
+    size_t j = 0;
+    while (j < names.size() && names[j] != s->labels[k]) j++;
+    if (j == names.size()) { names.push_back(s->labels[k]); sums.push_back(0.0); counts.push_back(0); }
+    sums[j] += t;
+    counts[j]++;
+  }
+  for (size_t j = 0; j < names.size(); j++) {
+    s->labels[j] = names[j];
+    if ((int)j < cap) { label_ms[j] = (float)sums[j]; label_count[j] = counts[j]; }
+  }
+  s->nmarks = (int)names.size();
+  if (nlabels) *nlabels = (int)names.size();
+  return XPBI_OK;
+}
+
+int xpbi_set_residual_monitor(xpbi_sim* h, int enabled) {
+  if (!h) return XPBI_ERR_ARG;
+  h->s.residOn = enabled != 0;
+  return XPBI_OK;
+}
+
+int xpbi_read_residuals(xpbi_sim* h, double* out, int cap, int* count) {
+  if (!h) return XPBI_ERR_ARG;
+  Sim* s = &h->s;
+  int r = xpbi_sync(h);
+  if (r) return r;
+  int n = s->residIters;
+  if (count) *count = n;
+  int m = n < cap ? n : cap;
+  if (m > 0) {
+    if (!out) return fail(s, XPBI_ERR_ARG, "out is null");
+    return d2h(s, out, s->residOut, 2ull * m * sizeof(double));
+  }
+  return XPBI_OK;
+}
+
+}  // extern "C"

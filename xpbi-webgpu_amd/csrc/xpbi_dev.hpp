@@ -1,0 +1,521 @@
+// Device-side types and math of the XPBI substep (gfx950).
+//
+// Restates the math library of the reference, web/js/shaders/common.js:62-396,
+// as HIP __device__ functions.  Layout in HBM is structure-of-arrays with
+// 16-byte lanes: a 3x3 matrix per particle is three planes (float4 a, float4 b,
+// float c) holding its nine column-major entries, so a wave reads each plane
+// with fully coalesced dwordx4 / dword loads.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace xpbi {
+
+typedef uint32_t u32;
+
+// ---------------------------------------------------------------- records
+struct DevMat {  // 64-B record, common.js:21-33
+  float mu, lam, density;
+  u32 kind;
+  float p0, p1, p2, p3;
+  float color[4];
+  float boundFriction, mass, pad0, pad1;
+};
+static_assert(sizeof(DevMat) == 64, "material record");
+
+enum { MAT_ELASTIC = 0, MAT_DP = 1, MAT_NACC = 2, MAT_VM = 3, MAT_HB = 4, MAT_SNOW = 5, MAT_WATER = 6 };
+
+// Kernel parameters, passed by value (lives in SGPRs).  The first block is the
+// Params uniform of common.js:35-60; the tail holds host-derived constants
+// (formed in f32 exactly as the per-pair expressions of common.js would).
+struct KP {
+  float ox, oy, oz, cell;
+  int gx, gy, gz;
+  u32 C;
+  float dminx, dminy, dminz, dt;
+  float dmaxx, dmaxy, dmaxz, h;
+  float grx, gry, grz, support;
+  float cpx, cpy, cpz, cR;
+  float cvx, cvy, cvz, cActive;
+  u32 N;
+  float V0, xsphC, omega;
+  float pbfEps, sCorrK, contactOmega, kDamp;
+  // derived
+  float supp2;   // support * support
+  float sigW;    // 21 / (16 pi h^3)            (common.js:67)
+  float poly6C;  // 315 / (64 pi H^9)           (common.js:86)
+  float spikyC;  // -45 / (pi H^6)              (common.js:93)
+  float w0;      // kernelPoly6(0.2 support)    (simkernels.js:465)
+  float poly6_0; // kernelPoly6(0)              (simkernels.js:416)
+  float alphaT;  // (1/V0) / dt^2               (simkernels.js:302)
+  float vmax;    // 0.45 support / dt           (simkernels.js:535)
+  float dvCap;   // 0.2 support / dt            (simkernels.js:361)
+  float minD;    // 0.75 h                      (simkernels.js:333)
+  u32 gxy;       // gx * gy
+};
+
+// ---------------------------------------------------------------- vectors
+template <class T> struct t3 { T x, y, z; };
+typedef t3<float> f3;
+typedef t3<double> d3;
+template <class T> __device__ __forceinline__ t3<T> mk3(T a, T b, T c) { return t3<T>{a, b, c}; }
+template <class T> __device__ __forceinline__ t3<T> splat3(T s) { return t3<T>{s, s, s}; }
+template <class T> __device__ __forceinline__ t3<T> operator+(t3<T> a, t3<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class T> __device__ __forceinline__ t3<T> operator-(t3<T> a, t3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class T> __device__ __forceinline__ t3<T> operator-(t3<T> a) { return {-a.x, -a.y, -a.z}; }
+template <class T> __device__ __forceinline__ t3<T> operator*(t3<T> a, T s) { return {a.x * s, a.y * s, a.z * s}; }
+template <class T> __device__ __forceinline__ t3<T> operator*(T s, t3<T> a) { return {s * a.x, s * a.y, s * a.z}; }
+template <class T> __device__ __forceinline__ t3<T> operator*(t3<T> a, t3<T> b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+template <class T> __device__ __forceinline__ t3<T> operator/(t3<T> a, T s) { return {a.x / s, a.y / s, a.z / s}; }
+template <class T> __device__ __forceinline__ t3<T> operator/(t3<T> a, t3<T> b) { return {a.x / b.x, a.y / b.y, a.z / b.z}; }
+template <class T> __device__ __forceinline__ T dot(t3<T> a, t3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float xsqrt(float a) { return sqrtf(a); }
+__device__ __forceinline__ double xsqrt(double a) { return sqrt(a); }
+__device__ __forceinline__ float xabs(float a) { return fabsf(a); }
+__device__ __forceinline__ double xabs(double a) { return fabs(a); }
+template <class T> __device__ __forceinline__ T len(t3<T> a) { return xsqrt(dot(a, a)); }
+template <class T> __device__ __forceinline__ t3<T> normalize(t3<T> a) { return a / len(a); }
+template <class T> __device__ __forceinline__ t3<T> cross(t3<T> a, t3<T> b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <class T> __device__ __forceinline__ T wmin(T a, T b) { return b < a ? b : a; }
+template <class T> __device__ __forceinline__ T wmax(T a, T b) { return a < b ? b : a; }
+template <class T> __device__ __forceinline__ T wclamp(T e, T lo, T hi) { return wmin(wmax(e, lo), hi); }
+__device__ __forceinline__ f3 xyz(float4 a) { return f3{a.x, a.y, a.z}; }
+__device__ __forceinline__ f3 vlog(f3 e) { return {logf(e.x), logf(e.y), logf(e.z)}; }
+__device__ __forceinline__ f3 vexp(f3 e) { return {expf(e.x), expf(e.y), expf(e.z)}; }
+__device__ __forceinline__ f3 vsqrt(f3 e) { return {sqrtf(e.x), sqrtf(e.y), sqrtf(e.z)}; }
+__device__ __forceinline__ f3 vmaxs(f3 e, float lo) { return {wmax(e.x, lo), wmax(e.y, lo), wmax(e.z, lo)}; }
+__device__ __forceinline__ f3 vclamp(f3 e, float lo, float hi) {
+  return {wclamp(e.x, lo, hi), wclamp(e.y, lo, hi), wclamp(e.z, lo, hi)};
+}
+template <class T> __device__ __forceinline__ T comp(t3<T> v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+template <class T> __device__ __forceinline__ void setc(t3<T>& v, int k, T s) {
+  if (k == 0) v.x = s; else if (k == 1) v.y = s; else v.z = s;
+}
+
+// column-major 3x3 (c[col])
+template <class T> struct t33 { t3<T> c[3]; };
+typedef t33<float> f33;
+typedef t33<double> d33;
+template <class T> __device__ __forceinline__ t33<T> mzero() { return t33<T>{{splat3<T>(0), splat3<T>(0), splat3<T>(0)}}; }
+template <class T> __device__ __forceinline__ t33<T> mident() {
+  return t33<T>{{mk3<T>(1, 0, 0), mk3<T>(0, 1, 0), mk3<T>(0, 0, 1)}};
+}
+template <class T> __device__ __forceinline__ t3<T> mul(const t33<T>& A, t3<T> v) {
+  return A.c[0] * v.x + A.c[1] * v.y + A.c[2] * v.z;
+}
+template <class T> __device__ __forceinline__ t33<T> mul(const t33<T>& A, const t33<T>& B) {
+  return t33<T>{{mul(A, B.c[0]), mul(A, B.c[1]), mul(A, B.c[2])}};
+}
+template <class T> __device__ __forceinline__ t33<T> transpose(const t33<T>& A) {
+  return t33<T>{{mk3(A.c[0].x, A.c[1].x, A.c[2].x), mk3(A.c[0].y, A.c[1].y, A.c[2].y),
+                 mk3(A.c[0].z, A.c[1].z, A.c[2].z)}};
+}
+template <class T> __device__ __forceinline__ t33<T> madd(const t33<T>& A, const t33<T>& B) {
+  return t33<T>{{A.c[0] + B.c[0], A.c[1] + B.c[1], A.c[2] + B.c[2]}};
+}
+template <class T> __device__ __forceinline__ t33<T> mscale(const t33<T>& A, T s) {
+  return t33<T>{{A.c[0] * s, A.c[1] * s, A.c[2] * s}};
+}
+template <class T> __device__ __forceinline__ t33<T> outer(t3<T> a, t3<T> b) {
+  return t33<T>{{a * b.x, a * b.y, a * b.z}};
+}
+template <class T> __device__ __forceinline__ T det3(const t33<T>& m) {
+  return m.c[0].x * (m.c[1].y * m.c[2].z - m.c[2].y * m.c[1].z) -
+         m.c[1].x * (m.c[0].y * m.c[2].z - m.c[2].y * m.c[0].z) +
+         m.c[2].x * (m.c[0].y * m.c[1].z - m.c[1].y * m.c[0].z);
+}
+template <class T, class S> __device__ __forceinline__ t33<T> mcast(const t33<S>& a) {
+  t33<T> r;
+  for (int j = 0; j < 3; j++) r.c[j] = mk3<T>(T(a.c[j].x), T(a.c[j].y), T(a.c[j].z));
+  return r;
+}
+template <class T, class S> __device__ __forceinline__ t3<T> vcast(t3<S> a) { return mk3<T>(T(a.x), T(a.y), T(a.z)); }
+
+// ---------------------------------------------------------------- planes
+struct M3P {  // nine floats per particle: a = (c0.xyz, c1.x), b = (c1.yz, c2.xy), c = c2.z
+  float4* a;
+  float4* b;
+  float* c;
+};
+struct M3C {
+  const float4* a;
+  const float4* b;
+  const float* c;
+};
+__device__ __forceinline__ f33 ldm(const float4* __restrict__ A, const float4* __restrict__ B,
+                                   const float* __restrict__ Cc, u32 i) {
+  float4 a = A[i], b = B[i];
+  float c = Cc[i];
+  return f33{{f3{a.x, a.y, a.z}, f3{a.w, b.x, b.y}, f3{b.z, b.w, c}}};
+}
+__device__ __forceinline__ f33 ldm(M3C m, u32 i) { return ldm(m.a, m.b, m.c, i); }
+__device__ __forceinline__ f33 ldm(M3P m, u32 i) { return ldm(m.a, m.b, m.c, i); }
+__device__ __forceinline__ void stm(M3P m, u32 i, const f33& v) {
+  m.a[i] = make_float4(v.c[0].x, v.c[0].y, v.c[0].z, v.c[1].x);
+  m.b[i] = make_float4(v.c[1].y, v.c[1].z, v.c[2].x, v.c[2].y);
+  m.c[i] = v.c[2].z;
+}
+
+// ---------------------------------------------------------------- SPH kernels
+// common.js:63-69
+__device__ __forceinline__ float kernelW(const KP& P, float r) {
+  float q = r / P.h;
+  if (q >= 2.0f) return 0.0f;
+  float t = 1.0f - 0.5f * q;
+  return P.sigW * t * t * t * t * (2.0f * q + 1.0f);
+}
+// common.js:71-79, dxv = xp - xb
+__device__ __forceinline__ f3 kernelGradW(const KP& P, f3 dxv) {
+  float r = len(dxv);
+  float q = r / P.h;
+  if (q >= 2.0f || r < 1e-9f) return splat3(0.0f);
+  float t = 1.0f - 0.5f * q;
+  float dwdq = P.sigW * (-5.0f * q) * t * t * t;
+  return dxv * (dwdq / (P.h * r));
+}
+// common.js:82-87
+__device__ __forceinline__ float kernelPoly6(const KP& P, float r) {
+  float H = P.support;
+  if (r >= H) return 0.0f;
+  float x = H * H - r * r;
+  return P.poly6C * x * x * x;
+}
+// common.js:88-94
+__device__ __forceinline__ f3 kernelSpikyGrad(const KP& P, f3 dxv) {
+  float H = P.support;
+  float r = len(dxv);
+  if (r >= H || r < 1e-9f) return splat3(0.0f);
+  float t = H - r;
+  return dxv * (P.spikyC * t * t / r);
+}
+
+// ---------------------------------------------------------------- grid hash
+// common.js:97-103.  IEEE f32 division (hipcc's correctly rounded default) and
+// floor; v_cvt_i32_f32 saturates and maps NaN to 0 like the WGSL conversion.
+struct i3 { int x, y, z; };
+__device__ __forceinline__ int f2i_sat(float f) { return __float2int_rz(f); }
+__device__ __forceinline__ i3 cellCoord(const KP& P, f3 p) {
+  int cx = f2i_sat(floorf((p.x - P.ox) / P.cell));
+  int cy = f2i_sat(floorf((p.y - P.oy) / P.cell));
+  int cz = f2i_sat(floorf((p.z - P.oz) / P.cell));
+  cx = min(max(cx, 0), P.gx - 1);
+  cy = min(max(cy, 0), P.gy - 1);
+  cz = min(max(cz, 0), P.gz - 1);
+  return i3{cx, cy, cz};
+}
+__device__ __forceinline__ u32 cellIndex(const KP& P, i3 c) {
+  return u32(c.x) + u32(c.y) * u32(P.gx) + u32(c.z) * P.gxy;
+}
+// pos.w holds the material index as a float value (simkernels.js:41-44); the
+// mask keeps a corrupt tag inside the 8-entry material table.
+__device__ __forceinline__ u32 matOfTag(float w) { return __float2uint_rz(w) & 7u; }
+
+// ---------------------------------------------------------------- svd3
+template <class T> struct SVD3 {
+  t33<T> U;
+  t3<T> sig;
+  t33<T> V;
+  bool degenerate;  // a fallback branch fired or sigma_min << sigma_max
+};
+template <class T> __device__ __forceinline__ T& el(t33<T>& m, int col, int row) {
+  return row == 0 ? m.c[col].x : (row == 1 ? m.c[col].y : m.c[col].z);
+}
+template <class T> __device__ __forceinline__ T elc(const t33<T>& m, int col, int row) {
+  return row == 0 ? m.c[col].x : (row == 1 ? m.c[col].y : m.c[col].z);
+}
+// common.js:108-143: one Jacobi rotation in the (p,q) plane, S <- J^T S J, V <- V J
+template <class T, int p, int q> __device__ __forceinline__ void jacobiRot(t33<T>& S, t33<T>& Vm) {
+  T Spq = elc(S, q, p);
+  T Spp = elc(S, p, p);
+  T Sqq = elc(S, q, q);
+  if (xabs(Spq) < T(1e-12)) return;
+  T tau = (Sqq - Spp) / (T(2) * Spq);
+  T rt = xsqrt(T(1) + tau * tau);
+  T t = T(1) / (tau >= T(0) ? tau + rt : tau - rt);
+  T c = T(1) / xsqrt(T(1) + t * t);
+  T s = t * c;
+  t33<T> Sn = S;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    T Skp = elc(S, p, k), Skq = elc(S, q, k);
+    el(Sn, p, k) = c * Skp - s * Skq;
+    el(Sn, q, k) = s * Skp + c * Skq;
+  }
+  t33<T> S2 = Sn;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    T Spk = elc(Sn, k, p), Sqk = elc(Sn, k, q);
+    el(S2, k, p) = c * Spk - s * Sqk;
+    el(S2, k, q) = s * Spk + c * Sqk;
+  }
+  S = S2;
+  t33<T> Vn = Vm;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    T Vkp = elc(Vm, p, k), Vkq = elc(Vm, q, k);
+    el(Vn, p, k) = c * Vkp - s * Vkq;
+    el(Vn, q, k) = s * Vkp + c * Vkq;
+  }
+  Vm = Vn;
+}
+template <class T> __device__ __forceinline__ void swp(T& a, T& b) { T t = a; a = b; b = t; }
+// common.js:145-180
+template <class T> __device__ __forceinline__ SVD3<T> svd3(const t33<T>& F) {
+  t33<T> S = mul(transpose(F), F);
+  t33<T> Vm = mident<T>();
+#pragma unroll 1
+  for (int sweep = 0; sweep < 6; sweep++) {
+    jacobiRot<T, 0, 1>(S, Vm);
+    jacobiRot<T, 0, 2>(S, Vm);
+    jacobiRot<T, 1, 2>(S, Vm);
+  }
+  t3<T> sig = mk3(xsqrt(wmax(S.c[0].x, T(0))), xsqrt(wmax(S.c[1].y, T(0))), xsqrt(wmax(S.c[2].z, T(0))));
+  t3<T> v0 = Vm.c[0], v1 = Vm.c[1], v2 = Vm.c[2];
+  if (sig.x < sig.y) { swp(sig.x, sig.y); swp(v0, v1); }
+  if (sig.x < sig.z) { swp(sig.x, sig.z); swp(v0, v2); }
+  if (sig.y < sig.z) { swp(sig.y, sig.z); swp(v1, v2); }
+  if (dot(cross(v0, v1), v2) < T(0)) v2 = -v2;
+  bool degen = false;
+  t3<T> u0, u1;
+  const T eps = T(1e-8);
+  if (sig.x > eps) u0 = normalize(mul(F, v0));
+  else { u0 = mk3<T>(1, 0, 0); degen = true; }
+  if (sig.y > eps) {
+    t3<T> Fv1 = mul(F, v1);
+    u1 = Fv1 - dot(Fv1, u0) * u0;
+    T l = len(u1);
+    if (l > eps) u1 = u1 / l;
+    else { u1 = normalize(cross(mk3<T>(0, 1, 0), u0) + splat3(T(1e-4))); degen = true; }
+  } else {
+    t3<T> a = cross(u0, mk3<T>(0, 1, 0));
+    if (len(a) < T(1e-4)) a = cross(u0, mk3<T>(1, 0, 0));
+    u1 = normalize(a);
+    degen = true;
+  }
+  t3<T> u2 = cross(u0, u1);
+  if (det3(F) < T(0)) sig.z = -sig.z;
+  SVD3<T> r;
+  r.U = t33<T>{{u0, u1, u2}};
+  r.sig = sig;
+  r.V = t33<T>{{v0, v1, v2}};
+  r.degenerate = degen;
+  return r;
+}
+// f32 SVD of F^T F loses sigma_min when sigma_min/sigma_max is small (the
+// condition number is squared).  That near-degenerate branch is recomputed in
+// FP64; everything else stays FP32.
+__device__ __forceinline__ SVD3<float> svd3_guarded(const f33& F) {
+  SVD3<float> s = svd3<float>(F);
+  float smin = fabsf(s.sig.z);
+  if (s.degenerate || smin < 0.02f * s.sig.x) {
+    SVD3<double> d = svd3<double>(mcast<double>(F));
+    s.U = mcast<float>(d.U);
+    s.V = mcast<float>(d.V);
+    s.sig = vcast<float>(d.sig);
+  }
+  return s;
+}
+// common.js:182-184  U diag(d) V^T
+template <class T> __device__ __forceinline__ t33<T> diagMul(const t33<T>& U, t3<T> d, const t33<T>& Vm) {
+  t33<T> A{{U.c[0] * d.x, U.c[1] * d.y, U.c[2] * d.z}};
+  return mul(A, transpose(Vm));
+}
+
+// ---------------------------------------------------------------- StVK-Hencky
+// common.js:187-201
+__device__ __forceinline__ f3 safeSig(f3 s) { return vclamp(s, 1e-3f, 1e3f); }
+__device__ __forceinline__ float stvkPsi(f3 sig, float mu, float lam) {
+  f3 e = vlog(safeSig(sig));
+  float tr = e.x + e.y + e.z;
+  return mu * dot(e, e) + 0.5f * lam * tr * tr;
+}
+__device__ __forceinline__ f3 stvkDPsi(f3 sig, float mu, float lam) {
+  f3 s = safeSig(sig);
+  f3 e = vlog(s);
+  float tr = e.x + e.y + e.z;
+  return (2.0f * mu * e + splat3(lam * tr)) / s;
+}
+
+// ---------------------------------------------------------------- return maps
+struct MatV {  // material in registers
+  float mu, lam, density;
+  u32 kind;
+  float p0, p1, p2, p3;
+  float fric, mass;
+};
+__device__ __forceinline__ MatV loadMat(const DevMat* m, u32 k) {
+  const DevMat& r = m[k];
+  return MatV{r.mu, r.lam, r.density, r.kind, r.p0, r.p1, r.p2, r.p3, r.boundFriction, r.mass};
+}
+// common.js:206-221
+__device__ __forceinline__ f3 rmDruckerPrager(f3 sig, const MatV& m) {
+  float alpha = m.p0, coh = m.p1;
+  f3 e = vlog(safeSig(sig)) - splat3(coh);
+  float tr = e.x + e.y + e.z;
+  f3 dev = e - splat3(tr / 3.0f);
+  float dn = len(dev);
+  if (dn < 1e-9f || tr > 0.0f) return vexp(splat3(coh));
+  float dg = dn + alpha * (3.0f * m.lam + 2.0f * m.mu) / (2.0f * m.mu) * tr;
+  if (dg <= 0.0f) return sig;
+  f3 en = e - (dg / dn) * dev;
+  return vexp(en + splat3(coh));
+}
+// common.js:224-234
+__device__ __forceinline__ f3 rmVonMises(f3 sig, const MatV& m) {
+  float sigY = m.p0;
+  f3 e = vlog(safeSig(sig));
+  float tr = e.x + e.y + e.z;
+  f3 dev = e - splat3(tr / 3.0f);
+  float dn = len(dev);
+  float dg = dn - sigY / (2.0f * m.mu);
+  if (dg <= 0.0f || dn < 1e-9f) return sig;
+  f3 en = e - (dg / dn) * dev;
+  return vexp(en);
+}
+// common.js:244-280
+__device__ __forceinline__ f3 rmHerschelBulkley(f3 sig, const MatV& m, float dt) {
+  float sigY = m.p0, hexp = m.p1, eta = m.p2;
+  f3 s3 = safeSig(sig);
+  float J = s3.x * s3.y * s3.z;
+  f3 bbar = s3 * s3 * powf(J, -2.0f / 3.0f);
+  float trb = bbar.x + bbar.y + bbar.z;
+  f3 devb = bbar - splat3(trb / 3.0f);
+  float sPre = m.mu * len(devb);
+  float sY = 0.816496580927726f * sigY;  // sqrt(2/3)
+  if (sPre <= sY || sPre < 1e-12f) return sig;
+  float muT = (trb / 3.0f) * m.mu;
+  float sNew;
+  if (fabsf(hexp - 1.0f) < 1e-3f || eta < 1e-9f) {
+    sNew = sPre - (sPre - sY) / (1.0f + eta / (2.0f * muT * dt));
+  } else {
+    float ih = 1.0f / hexp;
+    float ei = powf(eta, ih);
+    float lo = sY, hi = sPre;
+#pragma unroll 1
+    for (int it = 0; it < 18; it++) {
+      float mid = 0.5f * (lo + hi);
+      float g = ei * (mid - sPre) + 2.0f * muT * dt * powf(wmax(mid - sY, 0.0f), ih);
+      if (g > 0.0f) hi = mid; else lo = mid;
+    }
+    sNew = 0.5f * (lo + hi);
+  }
+  f3 bNew = devb * (sNew / sPre) + splat3(trb / 3.0f);
+  bNew = vmaxs(bNew, 1e-6f);
+  float detB = bNew.x * bNew.y * bNew.z;
+  bNew = bNew * powf(detB, -1.0f / 3.0f);
+  return vsqrt(bNew) * powf(J, 1.0f / 3.0f);
+}
+// common.js:283-285
+__device__ __forceinline__ f3 rmSnowClamp(f3 sig, const MatV& m) {
+  float lo = 1.0f - m.p0, hi = 1.0f + m.p1;
+  return f3{wclamp(sig.x, lo, hi), wclamp(sig.y, lo, hi), wclamp(sig.z, lo, hi)};
+}
+// common.js:293-364
+__device__ __noinline__ f3 rmNACC(f3 sig, const MatV m, float alphaIn, float* alphaOut) {
+  float beta = m.p1, xi = m.p2, Mf = m.p3;
+  float mu = m.mu;
+  float kappa = (2.0f / 3.0f) * mu + m.lam;
+  float alpha = alphaIn;
+  f3 s3 = safeSig(sig);
+  float Jtr = s3.x * s3.y * s3.z;
+  f3 b2 = s3 * s3;
+  float trb = b2.x + b2.y + b2.z;
+  f3 devb = b2 - splat3(trb / 3.0f);
+  f3 sHat = mu * powf(Jtr, -2.0f / 3.0f) * devb;
+  float sNorm = len(sHat);
+  const float sq15 = 1.2247448713915890f;  // sqrt(1.5)
+  float q = sq15 * sNorm;
+  float p = -0.5f * kappa * (Jtr * Jtr - 1.0f);
+  alpha = wclamp(alpha, -0.25f, 0.4f);
+  float p0 = kappa * (0.00001f + sinhf(wmin(xi * wmax(-alpha, 0.0f), 4.0f)));
+  float b = 1.0f + 2.0f * beta;
+  f3 outSig = sig;
+  if (p > p0) {
+    float Jn = sqrtf(wmax(-2.0f * p0 / kappa + 1.0f, 1e-6f));
+    outSig = splat3(powf(Jn, 1.0f / 3.0f));
+    alpha = alpha + logf(Jtr / Jn);
+  } else if (p < -beta * p0) {
+    float Jn = sqrtf(wmax(2.0f * beta * p0 / kappa + 1.0f, 1e-6f));
+    outSig = splat3(powf(Jn, 1.0f / 3.0f));
+    alpha = alpha + logf(Jtr / Jn);
+  } else {
+    float y = b * q * q + Mf * Mf * (p + beta * p0) * (p - p0);
+    if (y > 1e-4f && sNorm > 1e-9f) {
+      if (p0 > 1e-4f && p < p0 - 1e-4f && p > -beta * p0 + 1e-4f) {
+        float pc = (1.0f - beta) * p0 * 0.5f;
+        float dpc = pc - p, dq = -q;
+        float lenDir = sqrtf(dpc * dpc + dq * dq);
+        if (lenDir > 1e-9f) {
+          float Dp = (pc - p) / lenDir;
+          float Dq = -q / lenDir;
+          float Aq = Mf * Mf * Dp * Dp + b * Dq * Dq;
+          float Bq = Mf * Mf * Dp * (2.0f * pc - p0 + beta * p0);
+          float Cq = Mf * Mf * (pc + beta * p0) * (pc - p0);
+          float disc = Bq * Bq - 4.0f * Aq * Cq;
+          if (disc > 0.0f && fabsf(Aq) > 1e-12f) {
+            float l1 = (-Bq + sqrtf(disc)) / (2.0f * Aq);
+            float l2 = (-Bq - sqrtf(disc)) / (2.0f * Aq);
+            float px1 = pc + l1 * Dp;
+            float px2 = pc + l2 * Dp;
+            float px = px1;
+            if ((p - pc) * (px2 - pc) > 0.0f) px = px2;
+            float Jx2 = -2.0f * px / kappa + 1.0f;
+            if (Jx2 > 1e-8f) {
+              float Jx = sqrtf(Jx2);
+              if (Jx > 1e-4f) alpha = alpha + logf(Jtr / Jx);
+            }
+          }
+        }
+      }
+      float qN = Mf * sqrtf(wmax(-(p + beta * p0) * (p - p0), 0.0f) / b);
+      float sNewNorm = qN / sq15;
+      f3 bNew = (powf(Jtr, 2.0f / 3.0f) / mu) * (sNewNorm / sNorm) * sHat + splat3(trb / 3.0f);
+      bNew = vmaxs(bNew, 1e-6f);
+      outSig = vsqrt(bNew);
+    }
+  }
+  *alphaOut = alpha;
+  return outSig;
+}
+// common.js:367-375
+__device__ __forceinline__ MatV hardenedMat(MatV m, float state) {
+  if (m.kind == MAT_SNOW) {
+    float s = expf(wmin(m.p2 * (1.0f - state), 5.0f));
+    m.mu = m.mu * s;
+    m.lam = m.lam * s;
+  }
+  return m;
+}
+// common.js:379-396
+__device__ __forceinline__ f3 returnMap(f3 sig, const MatV& m, float state, float dt, float* stateOut) {
+  *stateOut = state;
+  switch (m.kind) {
+    case MAT_DP: return rmDruckerPrager(sig, m);
+    case MAT_NACC: return rmNACC(sig, m, state, stateOut);
+    case MAT_VM: return rmVonMises(sig, m);
+    case MAT_HB: return rmHerschelBulkley(sig, m, dt);
+    case MAT_SNOW: {
+      MatV mm = hardenedMat(m, state);
+      f3 sNew = rmSnowClamp(sig, mm);
+      float jOld = sig.x * sig.y * sig.z;
+      float jNew = sNew.x * sNew.y * sNew.z;
+      *stateOut = wclamp(state * jOld / wmax(jNew, 1e-6f), 0.2f, 2.0f);
+      return sNew;
+    }
+    default: return sig;
+  }
+}
+
+// ---------------------------------------------------------------- launch helpers
+// Blocks are dispatched round-robin over the 8 XCDs; remap so that each XCD
+// walks one contiguous run of particles (cell order), keeping the neighbour
+// rows of a block in that XCD's own L2.
+__device__ __forceinline__ u32 xcdBlock() {
+  u32 nb = gridDim.x, b = blockIdx.x;
+  u32 q = nb >> 3, r = nb & 7u, x = b & 7u, k = b >> 3;
+  return (x < r) ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
+}  // namespace xpbi

@@ -1,0 +1,100 @@
+// Host-side state of one simulation and the launchers shared by the .hip files.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+#include <vector>
+#include "xpbi_dev.hpp"
+#include "../../include/xpbi.h"
+
+namespace xpbi {
+
+typedef uint64_t u64;
+
+// particle arrays that are gathered into cell order every substep (two copies:
+// the reorder reads one set and writes the other)
+struct PSet {
+  float4* pos;
+  float4* Fa;
+  float4* Fb;
+  float* Fc;
+  float4* aux;
+  u32* id;
+};
+struct PSetC {
+  const float4* pos;
+  const float4* Fa;
+  const float4* Fb;
+  const float* Fc;
+  const float4* aux;
+  const u32* id;
+};
+
+constexpr int kMaxProfileMarks = 4096;
+constexpr int kMaxResidIters = 4096;
+
+struct Sim {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  u32 N = 0, C = 0;
+  int dims[3] = {0, 0, 0};
+  float origin[3] = {0, 0, 0};
+  float cell = 0;
+  double h = 0;
+  bool haveParams = false, haveMats = false;
+  xpbi_params params{};
+  xpbi_material mats[8]{};
+  KP kp{};
+  DevMat* dMats = nullptr;
+
+  PSet set[2]{};
+  int cur = 0;
+  float4* vel[2] = {nullptr, nullptr};
+  int vcur = 0;  // index of the current velocity ("velA")
+  M3P L{}, Sp{};
+  float4* selfDv = nullptr;
+  u32* cellStart = nullptr;  // C + 1
+  u32* count = nullptr;      // C
+  u32* keys[2] = {nullptr, nullptr};
+  u32* vals[2] = {nullptr, nullptr};
+  u32* perm = nullptr;        // points into vals[]
+  u32* sortedKeys = nullptr;  // points into keys[]
+  u32* scanPart = nullptr;
+  u32* radixHist = nullptr;
+  bool sorted = false;  // a substep has run (cellStart/perm valid)
+
+  // residual monitor
+  bool residOn = false;
+  double* residPart = nullptr;  // 2 * blocks
+  double* residOut = nullptr;   // 2 * kMaxResidIters
+  int residIters = 0;
+
+  // profiling
+  bool profiling = false;
+  std::vector<hipEvent_t> events;
+  std::vector<std::string> labels;
+  int nmarks = 0;
+  void prof_mark(const char* label) {
+    if (!profiling || nmarks + 1 >= (int)events.size()) return;
+    hipEventRecord(events[nmarks + 1], stream);
+    labels[nmarks] = label;
+    nmarks++;
+  }
+
+  std::vector<void*> allocs;
+  std::string err;
+};
+
+// launchers
+void launch_sort(Sim* s);
+void launch_reorder(Sim* s);
+void launch_exclusive_scan(Sim* s, const u32* in, u32 n, u32* out, bool write_total, u32 total);
+size_t scan_partials_needed(u64 n);
+size_t radix_hist_needed(u64 n);
+int radix_passes(u32 C);
+void launch_solve_stages(Sim* s, int iterations, bool water, int stop);
+
+}  // namespace xpbi
+
+struct xpbi_sim {
+  xpbi::Sim s;
+};

@@ -1,0 +1,272 @@
+// Grid hash, stable counting sort and cell-order reorder (gfx950).
+//
+// Replaces the reference's clearBuffer + countParticles + scanBlocks +
+// scanApply + scatter + stagePVA/applyPVA/stageF/applyF
+// (web/js/sim.js:223-254, web/js/shaders/simkernels.js:78-186).
+//
+//  * k_hash_count: cell id per particle (common.js:97-103) and the per-cell
+//    histogram (atomicAdd on integer counts: the counts are order independent).
+//  * exclusive scan of the counts -> cellStart[0..C] with cellStart[C] = N; the
+//    scan is a wave64 shuffle scan inside reduce-then-scan blocks, replacing the
+//    reference's O(NB^2) serial block-sum loop (simkernels.js:117-121).
+//  * the permutation is a stable LSD radix sort of (cell id, index) with 8-bit
+//    digits; each block ranks its tile with wave64 ballot match masks, so the
+//    order inside a cell is ascending original index -- deterministic, unlike
+//    the reference's atomicSub scatter (simkernels.js:138).
+//  * k_reorder gathers every particle array into cell order in one pass and
+//    fuses the per-particle work of scatter: V^n = V0 clamp(det F, 0.1, 10),
+//    v += dt g, lambda = 0 (simkernels.js:142-146).
+#include "xpbi_internal.hpp"
+
+namespace xpbi {
+
+constexpr int SCAN_T = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_TILE = SCAN_T * SCAN_ITEMS;  // 4096 elements per block
+constexpr int RADIX_T = 256;
+constexpr int RADIX_ROUNDS = 8;
+constexpr int RADIX_TILE = RADIX_T * RADIX_ROUNDS;  // 2048 keys per block
+
+__global__ __launch_bounds__(256) void k_fill_u32(u32* __restrict__ p, u32 n, u32 v) {
+  u32 i = blockIdx.x * 256u + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// countParticles (simkernels.js:79-84) + key output for the radix sort
+__global__ __launch_bounds__(256) void k_hash_count(KP P, const float4* __restrict__ pos, u32* __restrict__ keys,
+                                                    u32* __restrict__ count) {
+  u32 i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= P.N) return;
+  float4 p = pos[i];
+  u32 c = cellIndex(P, cellCoord(P, f3{p.x, p.y, p.z}));
+  keys[i] = c;
+  atomicAdd(&count[c], 1u);
+}
+
+// ---------------------------------------------------------------- scan
+__device__ __forceinline__ u32 wave_incl_scan(u32 v, u32 lane) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    u32 t = __shfl_up(v, o, 64);
+    if (lane >= (u32)o) v += t;
+  }
+  return v;
+}
+// block-wide exclusive scan of one value per thread (256 threads); returns the
+// exclusive prefix and writes the block total to *total
+__device__ __forceinline__ u32 block_excl_scan(u32 v, u32* total) {
+  __shared__ u32 s_w[4];
+  u32 lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+  u32 inc = wave_incl_scan(v, lane);
+  if (lane == 63u) s_w[w] = inc;
+  __syncthreads();
+  u32 off = 0, tot = 0;
+#pragma unroll
+  for (u32 k = 0; k < 4; k++) {
+    u32 s = s_w[k];
+    if (k < w) off += s;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return off + inc - v;
+}
+
+__global__ __launch_bounds__(SCAN_T) void k_scan_reduce(const u32* __restrict__ in, u32 n, u32* __restrict__ part) {
+  u32 base = blockIdx.x * (u32)SCAN_TILE + threadIdx.x * (u32)SCAN_ITEMS;
+  u32 s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; k++) {
+    u32 idx = base + k;
+    if (idx < n) s += in[idx];
+  }
+  u32 tot;
+  (void)block_excl_scan(s, &tot);
+  if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+// single block: exclusive scan of the nb block totals, in place
+__global__ __launch_bounds__(SCAN_T) void k_scan_partials(u32* __restrict__ part, u32 nb) {
+  u32 carry = 0;
+  for (u32 base = 0; base < nb; base += SCAN_T) {
+    u32 idx = base + threadIdx.x;
+    u32 v = idx < nb ? part[idx] : 0u;
+    u32 tot;
+    u32 ex = block_excl_scan(v, &tot);
+    if (idx < nb) part[idx] = carry + ex;
+    carry += tot;
+  }
+}
+// out[i] = exclusive prefix of in (in may alias out); out[n] = total if wtot
+__global__ __launch_bounds__(SCAN_T) void k_scan_down(const u32* in, u32 n, const u32* __restrict__ part, u32* out,
+                                                     int wtot, u32 total_if_known) {
+  u32 base = blockIdx.x * (u32)SCAN_TILE + threadIdx.x * (u32)SCAN_ITEMS;
+  u32 v[SCAN_ITEMS];
+  u32 s = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; k++) {
+    u32 idx = base + k;
+    v[k] = idx < n ? in[idx] : 0u;
+    s += v[k];
+  }
+  u32 tot;
+  u32 run = part[blockIdx.x] + block_excl_scan(s, &tot);
+#pragma unroll
+  for (int k = 0; k < SCAN_ITEMS; k++) {
+    u32 idx = base + k;
+    if (idx < n) out[idx] = run;
+    run += v[k];
+  }
+  if (wtot && blockIdx.x == gridDim.x - 1 && threadIdx.x == SCAN_T - 1) out[n] = total_if_known;
+}
+
+// ---------------------------------------------------------------- radix sort
+// digit histogram of one 2048-key tile -> bh[digit * nblk + blk]
+__global__ __launch_bounds__(RADIX_T) void k_radix_hist(const u32* __restrict__ keys, u32 n, u32 shift,
+                                                       u32* __restrict__ bh, u32 nblk) {
+  __shared__ u32 h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  u32 base = blockIdx.x * (u32)RADIX_TILE;
+#pragma unroll
+  for (int r = 0; r < RADIX_ROUNDS; r++) {
+    u32 idx = base + r * RADIX_T + threadIdx.x;
+    if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  bh[threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+}
+
+// stable scatter of one tile; off[digit * nblk + blk] = global start of that
+// digit's keys from this block.  vals == nullptr means vals[i] = i.
+__global__ __launch_bounds__(RADIX_T) void k_radix_scatter(const u32* __restrict__ kin, const u32* __restrict__ vin,
+                                                          u32* __restrict__ kout, u32* __restrict__ vout, u32 n,
+                                                          u32 shift, const u32* __restrict__ off, u32 nblk) {
+  __shared__ u32 s_run[256];
+  __shared__ u32 s_base[256];
+  __shared__ u32 s_wcnt[4][256];
+  const u32 tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+  s_run[tid] = 0;
+  s_base[tid] = off[tid * nblk + blockIdx.x];
+  const uint64_t ltmask = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
+  u32 base = blockIdx.x * (u32)RADIX_TILE;
+  for (int r = 0; r < RADIX_ROUNDS; r++) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) s_wcnt[k][tid] = 0;
+    __syncthreads();
+    u32 idx = base + r * RADIX_T + tid;
+    bool valid = idx < n;
+    u32 key = valid ? kin[idx] : 0u;
+    u32 val = valid ? (vin ? vin[idx] : idx) : 0u;
+    u32 d = (key >> shift) & 255u;
+    uint64_t m = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < 8; bit++) {
+      uint64_t bb = __ballot((d >> bit) & 1u);
+      m &= ((d >> bit) & 1u) ? bb : ~bb;
+    }
+    u32 rank = (u32)__popcll(m & ltmask);
+    if (valid && rank == 0) s_wcnt[w][d] = (u32)__popcll(m);
+    __syncthreads();
+    if (valid) {
+      u32 pre = s_run[d];
+      for (u32 k = 0; k < w; k++) pre += s_wcnt[k][d];
+      u32 dst = s_base[d] + pre + rank;
+      kout[dst] = key;
+      vout[dst] = val;
+    }
+    __syncthreads();
+    s_run[tid] += s_wcnt[0][tid] + s_wcnt[1][tid] + s_wcnt[2][tid] + s_wcnt[3][tid];
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- reorder
+__global__ __launch_bounds__(256) void k_reorder(KP P, const u32* __restrict__ perm, PSetC src,
+                                                 const float4* __restrict__ vsrc, PSet dst, float4* __restrict__ vdst) {
+  u32 j = xcdBlock() * 256u + threadIdx.x;
+  if (j >= P.N) return;
+  u32 i = perm[j];
+  float4 p = src.pos[i];
+  float4 v = vsrc[i];
+  float4 a = src.aux[i];
+  f33 F = ldm(src.Fa, src.Fb, src.Fc, i);
+  u32 id = src.id[i];
+  float Vn = P.V0 * wclamp(det3(F), 0.1f, 10.0f);
+  v.x = v.x + P.dt * P.grx;
+  v.y = v.y + P.dt * P.gry;
+  v.z = v.z + P.dt * P.grz;
+  v.w = Vn;
+  a.x = Vn;
+  a.y = 0.0f;
+  dst.pos[j] = p;
+  vdst[j] = v;
+  dst.aux[j] = a;
+  stm(M3P{dst.Fa, dst.Fb, dst.Fc}, j, F);
+  dst.id[j] = id;
+}
+
+// ---------------------------------------------------------------- host side
+static inline u32 cdiv(u64 a, u64 b) { return (u32)((a + b - 1) / b); }
+
+void launch_exclusive_scan(Sim* s, const u32* in, u32 n, u32* out, bool write_total, u32 total) {
+  u32 nb = cdiv(n, SCAN_TILE);
+  if (nb == 0) nb = 1;
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_T), 0, s->stream, in, n, s->scanPart);
+  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(SCAN_T), 0, s->stream, s->scanPart, nb);
+  hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_T), 0, s->stream, in, n, (const u32*)s->scanPart, out,
+                     write_total ? 1 : 0, total);
+}
+
+size_t scan_partials_needed(u64 n) { return (size_t)cdiv(n, SCAN_TILE) + 1; }
+size_t radix_hist_needed(u64 n) { return (size_t)256 * cdiv(n, RADIX_TILE); }
+
+int radix_passes(u32 C) {
+  u32 maxKey = C > 0 ? C - 1 : 0;
+  int bits = 0;
+  while (bits < 32 && (maxKey >> bits) != 0) bits++;
+  int p = (bits + 7) / 8;
+  return p < 1 ? 1 : p;
+}
+
+void launch_sort(Sim* s) {
+  const u32 N = s->N, C = s->C;
+  const u32 gN = cdiv(N, 256);
+  hipLaunchKernelGGL(k_fill_u32, dim3(cdiv(C, 256)), dim3(256), 0, s->stream, s->count, C, 0u);
+  s->prof_mark("clear");
+  hipLaunchKernelGGL(k_hash_count, dim3(gN), dim3(256), 0, s->stream, s->kp, (const float4*)s->set[s->cur].pos,
+                     s->keys[0], s->count);
+  s->prof_mark("countParticles");
+  launch_exclusive_scan(s, s->count, C, s->cellStart, true, N);
+  s->prof_mark("scanCells");
+  const u32 nblk = cdiv(N, RADIX_TILE);
+  const int passes = radix_passes(C);
+  const u32* vin = nullptr;
+  int kb = 0;
+  for (int p = 0; p < passes; p++) {
+    u32 shift = 8u * (u32)p;
+    hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(RADIX_T), 0, s->stream, (const u32*)s->keys[kb], N, shift,
+                       s->radixHist, nblk);
+    launch_exclusive_scan(s, s->radixHist, 256u * nblk, s->radixHist, false, 0u);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(RADIX_T), 0, s->stream, (const u32*)s->keys[kb], vin,
+                       s->keys[kb ^ 1], s->vals[kb ^ 1], N, shift, (const u32*)s->radixHist, nblk);
+    vin = s->vals[kb ^ 1];
+    kb ^= 1;
+  }
+  s->perm = s->vals[kb];
+  s->sortedKeys = s->keys[kb];
+  s->prof_mark("radixSort");
+}
+
+void launch_reorder(Sim* s) {
+  const u32 gN = cdiv(s->N, 256);
+  int src = s->cur, dst = s->cur ^ 1;
+  PSet& S = s->set[src];
+  PSetC sc{S.pos, S.Fa, S.Fb, S.Fc, S.aux, S.id};
+  hipLaunchKernelGGL(k_reorder, dim3(gN), dim3(256), 0, s->stream, s->kp, (const u32*)s->perm, sc,
+                     (const float4*)s->vel[s->vcur], s->set[dst], s->vel[s->vcur ^ 1]);
+  s->cur = dst;
+  s->vcur ^= 1;
+  s->prof_mark("reorder");
+}
+
+}  // namespace xpbi

@@ -1,0 +1,229 @@
+"""Python host surface of the substep engine, mirroring web/js/sim.js.
+
+``Simulation(scene)`` keeps the reference's method names and argument meaning:
+``writeMaterials`` (sim.js:84-99), ``writeParams`` (sim.js:159-186),
+``encodeSubstep``/``substep`` (sim.js:220-272), ``profileSubstep``
+(sim.js:199-215) and the readback hooks ``readPos``/``readVel``
+(main.js:291-312).  Everything runs through libxpbi.so; there is no CPU path.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._abi import FIELD, STAGE, XpbiError, check, lib
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float)) if a is not None else None
+
+
+def pack_materials(materials, V0):
+    """writeMaterials packing (sim.js:84-99): mu, lambda and mass from E, nu, density (float64)."""
+    arr = (_abi.Material * 8)()
+    for i, m in enumerate(materials):
+        E, nu = float(m["E"]), float(m["nu"])
+        r = arr[i]
+        r.mu = E / (2 * (1 + nu))
+        r.lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+        r.density = float(m["density"])
+        r.kind = int(m["kind"])
+        r.p0, r.p1 = float(m.get("p0", 0.0)), float(m.get("p1", 0.0))
+        r.p2, r.p3 = float(m.get("p2", 0.0)), float(m.get("p3", 0.0))
+        col = m.get("color", [0.0, 0.0, 0.0])
+        r.color[0], r.color[1], r.color[2], r.color[3] = col[0], col[1], col[2], 1.0
+        r.boundFriction = float(m.get("boundFriction", 0.4))
+        r.mass = float(m["density"]) * V0
+    return arr
+
+
+class Simulation:
+    """One scene resident on one GPU (the reference's ``new Simulation(device, scene)``)."""
+
+    def __init__(self, scene, device=0):
+        self.scene = scene
+        pos = np.ascontiguousarray(scene["positions"], dtype=np.float32).reshape(-1, 3)
+        self.N = pos.shape[0]
+        self.h = float(scene["h"])
+        self.support = 2 * self.h
+        self.V0 = self.h * self.h * self.h
+        self._keep = []
+        d = _abi.SceneDesc()
+        d.N = self.N
+        d.positions = _fptr(pos)
+        self._keep.append(pos)
+        tags = scene.get("tags")
+        if tags is not None:
+            tags = np.ascontiguousarray(tags, dtype=np.float32)
+            d.tags = _fptr(tags)
+            self._keep.append(tags)
+        mids = np.ascontiguousarray(scene.get("matIds", np.zeros(self.N)), dtype=np.uint32)
+        d.matIds = mids.ctypes.data_as(C.POINTER(C.c_uint32))
+        self._keep.append(mids)
+        if scene.get("states") is not None:
+            st = np.ascontiguousarray(scene["states"], dtype=np.float32)
+            d.states = _fptr(st)
+            self._keep.append(st)
+        if scene.get("velocities") is not None:
+            v = np.ascontiguousarray(scene["velocities"], dtype=np.float32).reshape(-1, 3)
+            d.velocities = _fptr(v)
+            self._keep.append(v)
+        d.h = self.h
+        for k in range(3):
+            d.domainMin[k] = float(scene["domainMin"][k])
+            d.domainMax[k] = float(scene["domainMax"][k])
+        d.device = device
+        h = C.c_void_p()
+        check(lib().xpbi_create(C.byref(d), C.byref(h)), None, "xpbi_create")
+        self._h = h
+        self._keep = None
+        o = (C.c_float * 3)()
+        cs = C.c_float()
+        dims = (C.c_int32 * 3)()
+        nc = C.c_uint32()
+        n = C.c_uint32()
+        check(lib().xpbi_grid_info(self._h, o, C.byref(cs), dims, C.byref(nc), C.byref(n)), self._h)
+        self.gridOrigin = list(o)
+        self.cellSize = cs.value
+        self.gridDims = list(dims)
+        self.numCells = nc.value
+        self.writeMaterials(scene["materials"])
+
+    # -- reference surface -------------------------------------------------
+    def writeMaterials(self, materials):
+        arr = pack_materials(materials, self.V0)
+        check(lib().xpbi_write_materials(self._h, arr, len(materials)), self._h, "writeMaterials")
+
+    def writeParams(self, p):
+        """sim.js:159-186.  p: dt, domainMin/Max, gravity, collider*, xsphC, omega, contactOmega,
+        pbfEps, sCorrK, time, kDamp (default 0.4)."""
+        r = _abi.Params()
+        for k in range(3):
+            r.gridOrigin[k] = self.gridOrigin[k]
+            r.gridDims[k] = self.gridDims[k]
+            r.domainMin[k] = p["domainMin"][k]
+            r.domainMax[k] = p["domainMax"][k]
+            r.gravity[k] = p["gravity"][k]
+            r.colliderPos[k] = p.get("colliderPos", [0.0, 0.0, 0.0])[k]
+            r.colliderVel[k] = p.get("colliderVel", [0.0, 0.0, 0.0])[k]
+        r.cellSize = self.cellSize
+        r.numCells = self.numCells
+        r.dt = p["dt"]
+        r.h = self.h
+        r.support = self.support
+        r.colliderR = p.get("colliderR", 0.0)
+        r.colliderActive = 1.0 if p.get("colliderActive", False) else 0.0
+        r.N = self.N
+        r.V0 = self.V0
+        r.xsphC, r.omega = p["xsphC"], p["omega"]
+        r.pbfEps, r.sCorrK = p["pbfEps"], p["sCorrK"]
+        r.time = p.get("time", 0.0)
+        r.contactOmega = p["contactOmega"]
+        r.kDamp = p.get("kDamp", 0.4)
+        self.params = r
+        check(lib().xpbi_write_params(self._h, C.byref(r)), self._h, "writeParams")
+
+    def encodeSubstep(self, iterations, hasWater):
+        check(lib().xpbi_substep(self._h, int(iterations), int(bool(hasWater))), self._h, "substep")
+
+    substep = encodeSubstep
+
+    def step(self, substeps, iterations, hasWater):
+        check(lib().xpbi_step(self._h, int(substeps), int(iterations), int(bool(hasWater))), self._h, "step")
+
+    def substep_partial(self, iterations, hasWater, stage):
+        st = STAGE[stage] if isinstance(stage, str) else int(stage)
+        check(lib().xpbi_substep_partial(self._h, int(iterations), int(bool(hasWater)), st), self._h,
+              "substep_partial")
+
+    def profileSubstep(self, iterations, hasWater, cap=4096):
+        ms = (C.c_float * cap)()
+        n = C.c_int()
+        check(lib().xpbi_profile_substep(self._h, int(iterations), int(bool(hasWater)), ms, cap, C.byref(n)),
+              self._h, "profileSubstep")
+        out = []
+        for k in range(min(n.value, cap)):
+            out.append((lib().xpbi_profile_label(self._h, k).decode(), ms[k]))
+        return out
+
+    def time_substeps(self, substeps, iterations, hasWater):
+        ms = C.c_float()
+        check(lib().xpbi_time_substeps(self._h, int(substeps), int(iterations), int(bool(hasWater)),
+                                       C.byref(ms)), self._h, "time_substeps")
+        return ms.value
+
+    def time_substeps_detail(self, substeps, iterations, hasWater, cap=64):
+        """Timed substeps with a HIP event after every launch: (total_ms, {label: (ms_sum, launches)})."""
+        tot = C.c_float()
+        ms = (C.c_float * cap)()
+        cnt = (C.c_int * cap)()
+        n = C.c_int()
+        check(lib().xpbi_time_substeps_detail(self._h, int(substeps), int(iterations), int(bool(hasWater)),
+                                              C.byref(tot), ms, cnt, cap, C.byref(n)), self._h,
+              "time_substeps_detail")
+        out = {}
+        for k in range(min(n.value, cap)):
+            out[lib().xpbi_profile_label(self._h, k).decode()] = (ms[k], cnt[k])
+        return tot.value, out
+
+    def sync(self):
+        check(lib().xpbi_sync(self._h), self._h, "sync")
+
+    # -- readback (main.js:289-313) ----------------------------------------
+    _SHAPES = {"POS": (4, np.float32), "VEL": (4, np.float32), "AUX": (4, np.float32), "F": (12, np.float32),
+               "L": (12, np.float32), "SP": (12, np.float32), "PERM": (1, np.uint32), "ID": (1, np.uint32)}
+
+    def read(self, field):
+        if field == "CELLSTART":
+            out = np.empty(self.numCells, dtype=np.uint32)
+        else:
+            w, dt = self._SHAPES[field]
+            out = np.empty((self.N, w) if w > 1 else self.N, dtype=dt)
+        check(lib().xpbi_read(self._h, FIELD[field], out.ctypes.data, out.nbytes), self._h, f"read {field}")
+        return out
+
+    def write(self, field, arr):
+        arr = np.ascontiguousarray(arr)
+        check(lib().xpbi_write(self._h, FIELD[field], arr.ctypes.data, arr.nbytes), self._h, f"write {field}")
+
+    def readPos(self):
+        return self.read("POS")
+
+    def readVel(self):
+        return self.read("VEL")
+
+    def set_residual_monitor(self, on):
+        check(lib().xpbi_set_residual_monitor(self._h, int(bool(on))), self._h)
+
+    def read_residuals(self, cap=4096):
+        out = np.zeros(2 * cap, dtype=np.float64)
+        n = C.c_int()
+        check(lib().xpbi_read_residuals(self._h, out.ctypes.data_as(C.POINTER(C.c_double)), cap, C.byref(n)),
+              self._h)
+        return out[: 2 * min(n.value, cap)].reshape(-1, 2)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().xpbi_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def params_from_scene(scene, dt=None, collider=None):
+    """The per-frame Params of main.js:386-402 for a scene: dt = (1/60) timeScale / substeps
+    unless the scene fixes dt."""
+    if dt is None:
+        dt = scene.get("dt") or (1.0 / 60.0) * scene["timeScale"] / scene["substeps"]
+    p = dict(dt=dt, domainMin=scene["domainMin"], domainMax=scene["domainMax"], gravity=scene["gravity"],
+             xsphC=scene["xsphC"], omega=scene["omega"], contactOmega=scene["contactOmega"],
+             pbfEps=scene["pbfEps"], sCorrK=scene["sCorrK"], time=0.0, kDamp=scene.get("kDamp", 0.4))
+    col = collider or scene.get("collider")
+    if col:
+        p.update(colliderPos=col["pos"], colliderR=col["R"], colliderVel=col.get("vel", [0, 0, 0]),
+                 colliderActive=True)
+    return p

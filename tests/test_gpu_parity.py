@@ -47,22 +47,24 @@ def _rows(a, b, floor):
 def check_field(name, gpu, o32, o64, floor, tol, label):
     """GPU vs the f32 oracle (rel per particle) with the f32 conditioning floor taken into account.
 
-    * the GPU is no farther from the f64 oracle than 3x the f32 oracle is (or tol), and
-    * 99.9% of particles match the f32 oracle within tol (every particle, when the f32
-      oracle itself is within tol of the f64 one).
+    * the GPU is no farther from the f64 oracle than 3x the f32 oracle is (or tol);
+    * when the f32 oracle is within tol of the f64 one everywhere, every particle of the GPU
+      is within tol of the f32 oracle;
+    * otherwise (ill-conditioned particles, where 1 ulp of input moves any f32 result past
+      tol) the GPU has at most 3x as many particles past tol as the f32 oracle has against
+      the f64 oracle.
     """
     e = _rows(gpu, o32, floor)
     e64 = _rows(gpu, o64, floor)
     n64 = _rows(o32, o64, floor)
-    q = float(np.quantile(e, 0.999)) if e.size else 0.0
-    print(f"  {name}: gpu-f32 max {e.max(initial=0):.2e} q999 {q:.2e} (>tol: {(e > tol).sum()}) | "
-          f"gpu-f64 max {e64.max(initial=0):.2e} | f32-f64 max {n64.max(initial=0):.2e} "
-          f"(>tol: {(n64 > tol).sum()})", flush=True)
+    bad, bad_o = int((e > tol).sum()), int((n64 > tol).sum())
+    print(f"  {name}: gpu-f32 max {e.max(initial=0):.2e} (>tol: {bad}) | gpu-f64 max {e64.max(initial=0):.2e} "
+          f"| f32-f64 max {n64.max(initial=0):.2e} (>tol: {bad_o})", flush=True)
     assert e64.max(initial=0) <= max(tol, 3.0 * n64.max(initial=0)), f"{label}: {name} farther from f64 than f32"
-    if n64.max(initial=0) <= tol:
-        assert e.max(initial=0) < tol, f"{label}: {name} rel err {e.max():.3e}"
+    if bad_o == 0:
+        assert bad == 0, f"{label}: {name} rel err {e.max():.3e}"
     else:
-        assert q < tol, f"{label}: {name} 99.9th percentile rel err {q:.3e}"
+        assert bad <= 3 * bad_o, f"{label}: {name} {bad} particles past tol (f32 oracle: {bad_o})"
     return float(e.max(initial=0))
 
 

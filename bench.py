@@ -129,6 +129,10 @@ def main():
         resid = dict(tol=1e-4, plain_iterations=it_plain, cap=args.residual_cap,
                      residual_at_20=(float(rel[19]) if len(rel) > 19 else None), learned_iterations=None)
 
+    copy_bytes = 1 << 30
+    copy_ms = sim.copy_bandwidth(copy_bytes, reps=5)
+    copy_GBs = 2 * copy_bytes / (copy_ms / 1e3) / 1e9  # read + write
+
     cpu = None
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
@@ -144,6 +148,7 @@ def main():
                    if args.config == "headline" else f"DP sand column 40x80x40 r=0.01 ({N} particles)",
                    "particles": N, "iterations": I, "dt": scene["dt"], "settle_substeps": args.settle},
         "roofline": roofline, "gather_kernels_roofline": gather, "kernels": kern,
+        "stream_copy_GBs": round(copy_GBs, 1),
         "jacobi_to_residual": resid, "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -184,6 +184,13 @@ int xpbi_time_substeps(xpbi_sim* sim, int substeps, int iterations, int has_wate
 int xpbi_time_substeps_detail(xpbi_sim* sim, int substeps, int iterations, int has_water, float* total_ms,
                               float* label_ms, int* label_count, int cap, int* nlabels);
 
+/* Stream-copy ceiling: copies `bytes` (rounded down to 16) between two device
+ * buffers allocated for the call with a float4 copy kernel, `reps` times, and
+ * returns the milliseconds of the last copy.  No reference counterpart; it is
+ * the measured bandwidth ceiling and the known-size kernel that HBM byte
+ * counters are calibrated against. */
+int xpbi_copy_bandwidth(xpbi_sim* sim, size_t bytes, int reps, float* ms);
+
 /* Residual monitor (no reference counterpart: the reference runs a fixed
  * iteration count, web/js/sim.js:256).  When enabled, every solveA reduces
  * r_k = ||C + alpha~ lambda + gamma dt dC.v||_2 over solid particles; after a

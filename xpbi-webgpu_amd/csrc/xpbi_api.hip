@@ -78,6 +78,9 @@ void build_kp(Sim* s) {
   k.dvCap = 0.2f * H / p.dt;
   k.minD = 0.75f * h;
   k.gxy = (u32)(p.gridDims[0] * p.gridDims[1]);
+  k.invH = 1.0f / h;
+  k.halfInvH = 0.5f / h;
+  k.gradC = -5.0f * k.sigW / (h * h);
   s->kp = k;
 }
 #pragma clang fp contract(on)
@@ -218,6 +221,8 @@ int xpbi_create(const xpbi_scene_desc* d, xpbi_sim** out) {
   A(dalloc(s, &s->Sp.b, N));
   A(dalloc(s, &s->Sp.c, N));
   A(dalloc(s, &s->selfDv, N));
+  A(dalloc(s, &s->nbrIdx, (size_t)s->nbrK * N));
+  A(dalloc(s, &s->nbrCnt, N));
   A(dalloc(s, &s->cellStart, (size_t)C + 1));
   A(dalloc(s, &s->count, C));
   size_t np = scan_partials_needed(C);
@@ -530,6 +535,36 @@ int xpbi_time_substeps_detail(xpbi_sim* h, int substeps, int iterations, int has
   }
   s->nmarks = (int)names.size();
   if (nlabels) *nlabels = (int)names.size();
+  return XPBI_OK;
+}
+
+int xpbi_copy_bandwidth(xpbi_sim* h, size_t bytes, int reps, float* ms) {
+  if (!h || !ms || reps < 1 || bytes < 16) return XPBI_ERR_ARG;
+  Sim* s = &h->s;
+  int r = xpbi_sync(h);
+  if (r) return r;
+  u64 n = bytes / 16;
+  float4 *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, n * 16) != hipSuccess) return fail(s, XPBI_ERR_NOMEM, "copy buffer");
+  if (hipMalloc(&b, n * 16) != hipSuccess) { hipFree(a); return fail(s, XPBI_ERR_NOMEM, "copy buffer"); }
+  hipMemsetAsync(a, 0, n * 16, s->stream);
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  float t = 0.0f;
+  for (int k = 0; k < reps; k++) {
+    hipEventRecord(e0, s->stream);
+    launch_copy(s, a, b, n);
+    hipEventRecord(e1, s->stream);
+  }
+  hipError_t e = hipEventSynchronize(e1);
+  hipEventElapsedTime(&t, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(a);
+  hipFree(b);
+  if (e != hipSuccess) return hip_fail(s, e, "copy");
+  *ms = t;
   return XPBI_OK;
 }
 

@@ -52,6 +52,9 @@ struct KP {
   float dvCap;   // 0.2 support / dt            (simkernels.js:361)
   float minD;    // 0.75 h                      (simkernels.js:333)
   u32 gxy;       // gx * gy
+  float invH;    // 1 / h
+  float halfInvH;  // 0.5 / h
+  float gradC;   // -5 sigma / h^2: grad W = dxv * gradC * (1 - q/2)^3
 };
 
 // ---------------------------------------------------------------- vectors
@@ -73,6 +76,12 @@ __device__ __forceinline__ float xsqrt(float a) { return sqrtf(a); }
 __device__ __forceinline__ double xsqrt(double a) { return sqrt(a); }
 __device__ __forceinline__ float xabs(float a) { return fabsf(a); }
 __device__ __forceinline__ double xabs(double a) { return fabs(a); }
+__device__ __forceinline__ float xrcp(float a) { return __builtin_amdgcn_rcpf(a); }
+__device__ __forceinline__ double xrcp(double a) { return 1.0 / a; }
+__device__ __forceinline__ float xrsq(float a) { return __builtin_amdgcn_rsqf(a); }
+__device__ __forceinline__ double xrsq(double a) { return 1.0 / sqrt(a); }
+__device__ __forceinline__ float fsqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
+__device__ __forceinline__ double fsqrt(double a) { return sqrt(a); }
 template <class T> __device__ __forceinline__ T len(t3<T> a) { return xsqrt(dot(a, a)); }
 template <class T> __device__ __forceinline__ t3<T> normalize(t3<T> a) { return a / len(a); }
 template <class T> __device__ __forceinline__ t3<T> cross(t3<T> a, t3<T> b) {
@@ -175,6 +184,24 @@ __device__ __forceinline__ f3 kernelGradW(const KP& P, f3 dxv) {
   float dwdq = P.sigW * (-5.0f * q) * t * t * t;
   return dxv * (dwdq / (P.h * r));
 }
+// Same kernels in closed form for the hot sweeps.  With q = r/h and t = 1 - q/2:
+//   dwdq / (h r) = sigma (-5 q) t^3 / (h r) = (-5 sigma / h^2) t^3,
+// so grad W = dxv * gradC * t^3 needs one sqrt and no division.  r2 = |dxv|^2 is
+// already known (< support^2) from the neighbour test; the r < 1e-9 case of the
+// reference returns zero.
+__device__ __forceinline__ f3 gradWfast(const KP& P, f3 dxv, float r2) {
+  float r = fsqrt(r2);
+  float t = 1.0f - r * P.halfInvH;
+  float c = (r2 < 1e-18f) ? 0.0f : P.gradC * (t * t * t);
+  return dxv * c;
+}
+__device__ __forceinline__ float kernelWfast(const KP& P, float r2) {
+  float q = fsqrt(r2) * P.invH;
+  float t = 1.0f - 0.5f * q;
+  float t2 = t * t;
+  return P.sigW * (t2 * t2) * (2.0f * q + 1.0f);
+}
+
 // common.js:82-87
 __device__ __forceinline__ float kernelPoly6(const KP& P, float r) {
   float H = P.support;
@@ -225,54 +252,54 @@ template <class T> __device__ __forceinline__ T& el(t33<T>& m, int col, int row)
 template <class T> __device__ __forceinline__ T elc(const t33<T>& m, int col, int row) {
   return row == 0 ? m.c[col].x : (row == 1 ? m.c[col].y : m.c[col].z);
 }
-// common.js:108-143: one Jacobi rotation in the (p,q) plane, S <- J^T S J, V <- V J
-template <class T, int p, int q> __device__ __forceinline__ void jacobiRot(t33<T>& S, t33<T>& Vm) {
-  T Spq = elc(S, q, p);
-  T Spp = elc(S, p, p);
-  T Sqq = elc(S, q, q);
+// Symmetric 3x3 held as its 6 distinct entries.
+template <class T> struct Sym3 { T a00, a11, a22, a01, a02, a12; };
+
+// common.js:108-143: one cyclic-Jacobi rotation in the (p,q) plane, S <- J^T S J,
+// V <- V J, with the same tau/t/c/s as the reference.  S is symmetric, so only
+// its 6 distinct entries are rotated and the annihilated S_pq is set to 0
+// (the reference recomputes it as rounding noise).
+template <class T>
+__device__ __forceinline__ void jrot(T& Spp, T& Sqq, T& Spq, T& Srp, T& Srq, t3<T>& vp, t3<T>& vq) {
   if (xabs(Spq) < T(1e-12)) return;
-  T tau = (Sqq - Spp) / (T(2) * Spq);
+  T tau = (Sqq - Spp) * xrcp(T(2) * Spq);
   T rt = xsqrt(T(1) + tau * tau);
-  T t = T(1) / (tau >= T(0) ? tau + rt : tau - rt);
-  T c = T(1) / xsqrt(T(1) + t * t);
+  T t = xrcp(tau >= T(0) ? tau + rt : tau - rt);
+  T c = xrsq(T(1) + t * t);
   T s = t * c;
-  t33<T> Sn = S;
-#pragma unroll
-  for (int k = 0; k < 3; k++) {
-    T Skp = elc(S, p, k), Skq = elc(S, q, k);
-    el(Sn, p, k) = c * Skp - s * Skq;
-    el(Sn, q, k) = s * Skp + c * Skq;
-  }
-  t33<T> S2 = Sn;
-#pragma unroll
-  for (int k = 0; k < 3; k++) {
-    T Spk = elc(Sn, k, p), Sqk = elc(Sn, k, q);
-    el(S2, k, p) = c * Spk - s * Sqk;
-    el(S2, k, q) = s * Spk + c * Sqk;
-  }
-  S = S2;
-  t33<T> Vn = Vm;
-#pragma unroll
-  for (int k = 0; k < 3; k++) {
-    T Vkp = elc(Vm, p, k), Vkq = elc(Vm, q, k);
-    el(Vn, p, k) = c * Vkp - s * Vkq;
-    el(Vn, q, k) = s * Vkp + c * Vkq;
-  }
-  Vm = Vn;
+  T tq = t * Spq;
+  Spp = Spp - tq;
+  Sqq = Sqq + tq;
+  Spq = T(0);
+  T rp = Srp, rq = Srq;
+  Srp = c * rp - s * rq;
+  Srq = s * rp + c * rq;
+  t3<T> a = vp, b = vq;
+  vp = c * a - s * b;
+  vq = s * a + c * b;
 }
 template <class T> __device__ __forceinline__ void swp(T& a, T& b) { T t = a; a = b; b = t; }
-// common.js:145-180
+template <class T> struct JTol;
+template <> struct JTol<float> { static constexpr float v = 1e-7f; };
+template <> struct JTol<double> { static constexpr double v = 1e-15; };
+// common.js:145-180.  The reference runs 6 fixed sweeps; sweeps stop early here
+// once the off-diagonal mass is below the type's rounding level (a further
+// sweep would only move rounding noise).
 template <class T> __device__ __forceinline__ SVD3<T> svd3(const t33<T>& F) {
-  t33<T> S = mul(transpose(F), F);
-  t33<T> Vm = mident<T>();
+  const t3<T> f0 = F.c[0], f1 = F.c[1], f2 = F.c[2];
+  T a00 = dot(f0, f0), a11 = dot(f1, f1), a22 = dot(f2, f2);
+  T a01 = dot(f0, f1), a02 = dot(f0, f2), a12 = dot(f1, f2);
+  t3<T> w0 = mk3<T>(1, 0, 0), w1 = mk3<T>(0, 1, 0), w2 = mk3<T>(0, 0, 1);
 #pragma unroll 1
   for (int sweep = 0; sweep < 6; sweep++) {
-    jacobiRot<T, 0, 1>(S, Vm);
-    jacobiRot<T, 0, 2>(S, Vm);
-    jacobiRot<T, 1, 2>(S, Vm);
+    jrot(a00, a11, a01, a02, a12, w0, w1);  // (0,1): third index 2
+    jrot(a00, a22, a02, a01, a12, w0, w2);  // (0,2): third index 1
+    jrot(a11, a22, a12, a01, a02, w1, w2);  // (1,2): third index 0
+    T off = xabs(a01) + xabs(a02) + xabs(a12);
+    if (off <= JTol<T>::v * (a00 + a11 + a22)) break;
   }
-  t3<T> sig = mk3(xsqrt(wmax(S.c[0].x, T(0))), xsqrt(wmax(S.c[1].y, T(0))), xsqrt(wmax(S.c[2].z, T(0))));
-  t3<T> v0 = Vm.c[0], v1 = Vm.c[1], v2 = Vm.c[2];
+  t3<T> sig = mk3(xsqrt(wmax(a00, T(0))), xsqrt(wmax(a11, T(0))), xsqrt(wmax(a22, T(0))));
+  t3<T> v0 = w0, v1 = w1, v2 = w2;
   if (sig.x < sig.y) { swp(sig.x, sig.y); swp(v0, v1); }
   if (sig.x < sig.z) { swp(sig.x, sig.z); swp(v0, v2); }
   if (sig.y < sig.z) { swp(sig.y, sig.z); swp(v1, v2); }
@@ -280,13 +307,15 @@ template <class T> __device__ __forceinline__ SVD3<T> svd3(const t33<T>& F) {
   bool degen = false;
   t3<T> u0, u1;
   const T eps = T(1e-8);
-  if (sig.x > eps) u0 = normalize(mul(F, v0));
-  else { u0 = mk3<T>(1, 0, 0); degen = true; }
+  if (sig.x > eps) {
+    t3<T> Fv0 = mul(F, v0);
+    u0 = Fv0 * xrsq(dot(Fv0, Fv0));
+  } else { u0 = mk3<T>(1, 0, 0); degen = true; }
   if (sig.y > eps) {
     t3<T> Fv1 = mul(F, v1);
     u1 = Fv1 - dot(Fv1, u0) * u0;
-    T l = len(u1);
-    if (l > eps) u1 = u1 / l;
+    T l2 = dot(u1, u1);
+    if (l2 > eps * eps) u1 = u1 * xrsq(l2);
     else { u1 = normalize(cross(mk3<T>(0, 1, 0), u0) + splat3(T(1e-4))); degen = true; }
   } else {
     t3<T> a = cross(u0, mk3<T>(0, 1, 0));

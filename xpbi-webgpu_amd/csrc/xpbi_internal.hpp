@@ -60,6 +60,11 @@ struct Sim {
   u32* sortedKeys = nullptr;  // points into keys[]
   u32* scanPart = nullptr;
   u32* radixHist = nullptr;
+  // support-neighbour list of the current substep, built by computeL:
+  // nbrIdx[k * N + i] for k < nbrCnt[i] <= nbrK (nbrCnt[i] > nbrK: use the cell rows)
+  u32* nbrIdx = nullptr;
+  u32* nbrCnt = nullptr;
+  u32 nbrK = 64;
   bool sorted = false;  // a substep has run (cellStart/perm valid)
 
   // residual monitor
@@ -92,6 +97,7 @@ size_t scan_partials_needed(u64 n);
 size_t radix_hist_needed(u64 n);
 int radix_passes(u32 C);
 void launch_solve_stages(Sim* s, int iterations, bool water, int stop);
+void launch_copy(Sim* s, const float4* src, float4* dst, u64 n);
 
 }  // namespace xpbi
 

@@ -4,8 +4,11 @@
 // the dispatch order of web/js/sim.js:255-266:
 //   computeL -> [solveA -> solveB (-> pbfA -> pbfB)] x I -> xsph -> updateF -> integrate
 // One thread per particle in cell order (256-thread blocks, XCD-contiguous block
-// mapping).  Each neighbour sweep walks the 9 row-merged cell spans of the 27-cell
-// stencil (simkernels.js:62-74) directly out of the cell-ordered arrays.
+// mapping).  Positions are fixed for the whole substep, so the support
+// neighbours (b != i, |x_i - x_b| < 2h, in the reference's candidate order over
+// the 9 row-merged spans of simkernels.js:62-74) are found once, by computeL,
+// and stored as a slot-major list; every later sweep walks that list instead of
+// re-testing the ~216 candidates of the 27-cell stencil.
 #include "xpbi_internal.hpp"
 
 namespace xpbi {
@@ -34,16 +37,53 @@ __device__ __forceinline__ void forRows(const KP& P, const u32* __restrict__ cs,
   }
 }
 
+struct NLc {
+  const u32* __restrict__ idx;  // idx[k * N + i]
+  const u32* __restrict__ cnt;
+  u32 K, N;
+};
+
+// fn(b, pos[b], dxv = x_i - x_b, |dxv|^2) for each support neighbour of i.  A
+// particle whose list overflowed (cnt > K) walks the cell rows instead.
+template <class Fn>
+__device__ __forceinline__ void forSupport(const KP& P, const u32* __restrict__ cs, const float4* __restrict__ pos,
+                                           const NLc& nl, u32 i, f3 xi, Fn&& fn) {
+  const u32 n = nl.cnt[i];
+  if (n <= nl.K) {
+    const u32* __restrict__ p = nl.idx + i;
+#pragma unroll 2
+    for (u32 k = 0; k < n; k++) {
+      const u32 b = p[(size_t)k * nl.N];
+      const float4 pb = pos[b];
+      const f3 d = xi - xyz(pb);
+      fn(b, pb, d, dot(d, d));
+    }
+  } else {
+    forRows(P, cs, cellCoord(P, xi), [&](u32 b) {
+      if (b != i) {
+        const float4 pb = pos[b];
+        const f3 d = xi - xyz(pb);
+        const float r2 = dot(d, d);
+        if (r2 < P.supp2) fn(b, pb, d, r2);
+      }
+    });
+  }
+}
+
 // material tables staged in LDS at kernel entry
 struct MatLds {
   DevMat m[8];
   float mass[8];
+  float invMass[8];
   u32 solidMask;  // bit k set when material k is not water
 };
 __device__ __forceinline__ void loadMatLds(MatLds& t, const DevMat* __restrict__ mats) {
   const u32 tid = threadIdx.x;
   if (tid < 128) ((float*)t.m)[tid] = ((const float*)mats)[tid];
-  if (tid < 8) t.mass[tid] = mats[tid].mass;
+  if (tid < 8) {
+    t.mass[tid] = mats[tid].mass;
+    t.invMass[tid] = 1.0f / mats[tid].mass;
+  }
   if (tid == 0) {
     u32 mask = 0;
     for (int k = 0; k < 8; k++)
@@ -60,23 +100,30 @@ __device__ __forceinline__ bool solidTag(u32 mask, float w) { return (mask >> ma
 
 // ---------------------------------------------------------------- computeL
 // simkernels.js:189-221.  A is accumulated in f32; the SVD pseudo-inverse with
-// absolute tolerance 0.2 runs in FP64.
+// absolute tolerance 0.2 runs in FP64.  The same candidate walk records the
+// support-neighbour list of the substep.
 __global__ __launch_bounds__(256) void k_computeL(KP P, const float4* __restrict__ pos, const float4* __restrict__ vin,
-                                                  const u32* __restrict__ cs, M3P L) {
+                                                  const u32* __restrict__ cs, M3P L, u32* __restrict__ nIdx,
+                                                  u32* __restrict__ nCnt, u32 K) {
   u32 i = xcdBlock() * 256u + threadIdx.x;
   if (i >= P.N) return;
   f3 xi = xyz(pos[i]);
   f33 A = mzero<float>();
+  u32 n = 0;
   forRows(P, cs, cellCoord(P, xi), [&](u32 b) {
     if (b != i) {
       f3 dxv = xi - xyz(pos[b]);
-      if (dot(dxv, dxv) < P.supp2) {
-        f3 g = kernelGradW(P, dxv);
+      float r2 = dot(dxv, dxv);
+      if (r2 < P.supp2) {
+        f3 g = gradWfast(P, dxv, r2);
         f3 r = -dxv;
         A = madd(A, mscale(outer(g, r), vin[b].w));
+        if (n < K) nIdx[(size_t)n * P.N + i] = b;
+        n++;
       }
     }
   });
+  nCnt[i] = n;
   SVD3<double> s = svd3<double>(mcast<double>(A));
   d3 inv = splat3(0.0);
   if (fabs(s.sig.x) > 0.2) inv.x = 1.0 / s.sig.x;
@@ -89,18 +136,14 @@ __global__ __launch_bounds__(256) void k_computeL(KP P, const float4* __restrict
 // simkernels.js:224-245
 __device__ __forceinline__ f33 velocityGradient(const KP& P, const u32* __restrict__ cs,
                                                 const float4* __restrict__ pos, const float4* __restrict__ vin,
-                                                u32 solidMask, u32 i, f3 xi, f3 vi, const f33& Li, i3 cc) {
+                                                const NLc& nl, u32 solidMask, u32 i, f3 xi, f3 vi, const f33& Li) {
   f33 G = mzero<float>();
-  forRows(P, cs, cc, [&](u32 b) {
-    float4 pb = pos[b];
-    if (b != i && solidTag(solidMask, pb.w)) {
-      f3 dxv = xi - xyz(pb);
-      if (dot(dxv, dxv) < P.supp2) {
-        f3 gt = mul(Li, kernelGradW(P, dxv));
-        float4 vb = vin[b];
-        f3 dv = (xyz(vb) - vi) * vb.w;
-        G = madd(G, outer(dv, gt));
-      }
+  forSupport(P, cs, pos, nl, i, xi, [&](u32 b, float4 pb, f3 dxv, float r2) {
+    if (solidTag(solidMask, pb.w)) {
+      f3 gt = mul(Li, gradWfast(P, dxv, r2));
+      float4 vb = vin[b];
+      f3 dv = (xyz(vb) - vi) * vb.w;
+      G = madd(G, outer(dv, gt));
     }
   });
   return G;
@@ -128,7 +171,7 @@ __device__ __forceinline__ void blockReduce2(double a, double b, double* __restr
 __global__ __launch_bounds__(256) void k_solveA(KP P, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
                                                 const float4* __restrict__ vin, M3C Fm, M3C Lm,
                                                 float4* __restrict__ aux, M3P Sp, float4* __restrict__ selfDv,
-                                                const u32* __restrict__ cs, double* __restrict__ part) {
+                                                const u32* __restrict__ cs, NLc nl, double* __restrict__ part) {
   __shared__ MatLds T;
   loadMatLds(T, mats);
   const u32 i = xcdBlock() * 256u + threadIdx.x;
@@ -145,8 +188,7 @@ __global__ __launch_bounds__(256) void k_solveA(KP P, const DevMat* __restrict__
       const f33 Li = ldm(Lm, i);
       const float4 vi4 = vin[i];
       const f3 vi = xyz(vi4);
-      const i3 cc = cellCoord(P, xi);
-      f33 gradV = velocityGradient(P, cs, pos, vin, T.solidMask, i, xi, vi, Li, cc);
+      f33 gradV = velocityGradient(P, cs, pos, vin, nl, T.solidMask, i, xi, vi, Li);
       f33 Ftr = mul(madd(mident<float>(), mscale(gradV, P.dt)), Fn);
       SVD3<float> sv = svd3_guarded(Ftr);
       float st;
@@ -159,17 +201,13 @@ __global__ __launch_bounds__(256) void k_solveA(KP P, const DevMat* __restrict__
         f33 GL = mul(mul(Pk, transpose(Fn)), Li);
         float denom = 0.0f, cdot = 0.0f;
         f3 sgrad = splat3(0.0f);
-        forRows(P, cs, cc, [&](u32 b) {
-          float4 pb = pos[b];
-          if (b != i && solidTag(T.solidMask, pb.w)) {
-            f3 dxv = xi - xyz(pb);
-            if (dot(dxv, dxv) < P.supp2) {
-              float4 vb = vin[b];
-              f3 gb = vb.w * mul(GL, kernelGradW(P, dxv));
-              denom = denom + dot(gb, gb) / T.mass[matOfTag(pb.w)];
-              sgrad = sgrad - gb;
-              cdot = cdot + dot(gb, xyz(vb));
-            }
+        forSupport(P, cs, pos, nl, i, xi, [&](u32 b, float4 pb, f3 dxv, float r2) {
+          if (solidTag(T.solidMask, pb.w)) {
+            float4 vb = vin[b];
+            f3 gb = vb.w * mul(GL, gradWfast(P, dxv, r2));
+            denom = denom + dot(gb, gb) * T.invMass[matOfTag(pb.w)];
+            sgrad = sgrad - gb;
+            cdot = cdot + dot(gb, xyz(vb));
           }
         });
         denom = denom + dot(sgrad, sgrad) / m.mass;
@@ -246,7 +284,7 @@ __device__ __forceinline__ f3 applyBoundary(const KP& P, f3 xcIn, f3 vIn, f3 xn,
 __global__ __launch_bounds__(256) void k_solveB(KP P, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
                                                 const float4* __restrict__ vin, M3C Sp,
                                                 const float4* __restrict__ selfDv, const u32* __restrict__ cs,
-                                                float4* __restrict__ vout) {
+                                                NLc nl, float4* __restrict__ vout) {
   __shared__ MatLds T;
   loadMatLds(T, mats);
   const u32 i = xcdBlock() * 256u + threadIdx.x;
@@ -263,21 +301,20 @@ __global__ __launch_bounds__(256) void k_solveB(KP P, const DevMat* __restrict__
     const float Vi = vi4.w;
     const f3 xci = xi + P.dt * xyz(vi4);
     const float minD = P.minD;
-    forRows(P, cs, cellCoord(P, xi), [&](u32 b) {
-      float4 pb4 = pos[b];
-      if (b != i && solidTag(T.solidMask, pb4.w)) {
+    forSupport(P, cs, pos, nl, i, xi, [&](u32 b, float4 pb4, f3 dxv, float r2) {
+      if (solidTag(T.solidMask, pb4.w)) {
         f3 pb = xyz(pb4);
-        f3 dxp = pb - xi;
-        if (dot(dxp, dxp) < P.supp2) {
-          dv = dv + Vi * mul(ldm(Sp, b), kernelGradW(P, dxp));
-          f3 xcb = pb + P.dt * xyz(vin[b]);
-          f3 dxc = xci - xcb;
-          float dist = len(dxc);
-          if (dist < minD && dist > 1e-9f) {
-            float mb = T.mass[matOfTag(pb4.w)];
-            float w = mb / (mass_i + mb);
-            corr = corr + dxc / dist * ((minD - dist) * w);
-          }
+        f3 dxp = -dxv;  // x_b - x_i
+        dv = dv + Vi * mul(ldm(Sp, b), gradWfast(P, dxp, r2));
+        f3 xcb = pb + P.dt * xyz(vin[b]);
+        f3 dxc = xci - xcb;
+        float d2 = dot(dxc, dxc);
+        if (d2 < minD * minD && d2 > 1e-18f) {
+          float idist = xrsq(d2);
+          float dist = d2 * idist;
+          float mb = T.mass[matOfTag(pb4.w)];
+          float w = mb / (mass_i + mb);
+          corr = corr + dxc * (idist * ((minD - dist) * w));
         }
       }
     });
@@ -370,7 +407,7 @@ __global__ __launch_bounds__(256) void k_pbfB(KP P, const DevMat* __restrict__ m
 // ---------------------------------------------------------------- XSPH
 // simkernels.js:480-507
 __global__ __launch_bounds__(256) void k_xsph(KP P, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
-                                              const float4* __restrict__ vin, const u32* __restrict__ cs,
+                                              const float4* __restrict__ vin, const u32* __restrict__ cs, NLc nl,
                                               float4* __restrict__ vout) {
   __shared__ MatLds T;
   loadMatLds(T, mats);
@@ -382,14 +419,10 @@ __global__ __launch_bounds__(256) void k_xsph(KP P, const DevMat* __restrict__ m
   const f3 vi = xyz(vi4);
   const bool iSolid = solidTag(T.solidMask, pi.w);
   f3 acc = splat3(0.0f);
-  forRows(P, cs, cellCoord(P, xi), [&](u32 b) {
-    float4 pb = pos[b];
-    if (b != i && (solidTag(T.solidMask, pb.w) == iSolid)) {
-      f3 dxv = xi - xyz(pb);
-      if (dot(dxv, dxv) < P.supp2) {
-        float4 vb = vin[b];
-        acc = acc + vb.w * (xyz(vb) - vi) * kernelW(P, len(dxv));
-      }
+  forSupport(P, cs, pos, nl, i, xi, [&](u32 b, float4 pb, f3 dxv, float r2) {
+    if (solidTag(T.solidMask, pb.w) == iSolid) {
+      float4 vb = vin[b];
+      acc = acc + vb.w * (xyz(vb) - vi) * kernelWfast(P, r2);
     }
   });
   float c = P.xsphC;
@@ -402,7 +435,7 @@ __global__ __launch_bounds__(256) void k_xsph(KP P, const DevMat* __restrict__ m
 // simkernels.js:510-525
 __global__ __launch_bounds__(256) void k_updateF(KP P, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
                                                  const float4* __restrict__ vin, M3P Fm, M3C Lm,
-                                                 float4* __restrict__ aux, const u32* __restrict__ cs) {
+                                                 float4* __restrict__ aux, const u32* __restrict__ cs, NLc nl) {
   __shared__ MatLds T;
   loadMatLds(T, mats);
   const u32 i = xcdBlock() * 256u + threadIdx.x;
@@ -414,7 +447,7 @@ __global__ __launch_bounds__(256) void k_updateF(KP P, const DevMat* __restrict_
   MatV m = hardenedMat(matv(T, mi), state);
   const f3 xi = xyz(pi);
   const f33 Li = ldm(Lm, i);
-  f33 gradV = velocityGradient(P, cs, pos, vin, T.solidMask, i, xi, xyz(vin[i]), Li, cellCoord(P, xi));
+  f33 gradV = velocityGradient(P, cs, pos, vin, nl, T.solidMask, i, xi, xyz(vin[i]), Li);
   f33 Ftr = mul(madd(mident<float>(), mscale(gradV, P.dt)), ldm(Fm, i));
   SVD3<float> sv = svd3_guarded(Ftr);
   float st;
@@ -452,7 +485,8 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
   hipStream_t st = s->stream;
   int vi = s->vcur;
   hipLaunchKernelGGL(k_computeL, dim3(gN), dim3(256), 0, st, P, (const float4*)S.pos, (const float4*)s->vel[vi],
-                     (const u32*)s->cellStart, s->L);
+                     (const u32*)s->cellStart, s->L, s->nbrIdx, s->nbrCnt, s->nbrK);
+  const NLc nl{s->nbrIdx, s->nbrCnt, s->nbrK, s->N};
   s->prof_mark("computeL");
   if (stop == XPBI_STAGE_COMPUTE_L) return;
   s->residIters = 0;
@@ -460,7 +494,7 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
     double* part = (s->residOn && k < kMaxResidIters) ? s->residPart : nullptr;
     hipLaunchKernelGGL(k_solveA, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
                        (const float4*)s->vel[vi], cm(F), cm(s->L), S.aux, s->Sp, s->selfDv,
-                       (const u32*)s->cellStart, part);
+                       (const u32*)s->cellStart, nl, part);
     s->prof_mark("solveA");
     if (part) {
       hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(256), 0, st, (const double*)part, gN, s->residOut + 2 * k);
@@ -468,7 +502,7 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
     }
     hipLaunchKernelGGL(k_solveB, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
                        (const float4*)s->vel[vi], cm(s->Sp), (const float4*)s->selfDv, (const u32*)s->cellStart,
-                       s->vel[vi ^ 1]);
+                       nl, s->vel[vi ^ 1]);
     s->prof_mark("solveB");
     vi ^= 1;
     if (water) {
@@ -483,12 +517,12 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
   }
   if (stop == XPBI_STAGE_SOLVE) { s->vcur = vi; return; }
   hipLaunchKernelGGL(k_xsph, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
-                     (const float4*)s->vel[vi], (const u32*)s->cellStart, s->vel[vi ^ 1]);
+                     (const float4*)s->vel[vi], (const u32*)s->cellStart, nl, s->vel[vi ^ 1]);
   s->prof_mark("xsph");
   vi ^= 1;
   if (stop == XPBI_STAGE_XSPH) { s->vcur = vi; return; }
   hipLaunchKernelGGL(k_updateF, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
-                     (const float4*)s->vel[vi], F, cm(s->L), S.aux, (const u32*)s->cellStart);
+                     (const float4*)s->vel[vi], F, cm(s->L), S.aux, (const u32*)s->cellStart, nl);
   s->prof_mark("updateF");
   if (stop == XPBI_STAGE_UPDATE_F) { s->vcur = vi; return; }
   hipLaunchKernelGGL(k_integrate, dim3(gN), dim3(256), 0, st, P, S.pos, s->vel[vi]);

@@ -205,6 +205,13 @@ __global__ __launch_bounds__(256) void k_reorder(KP P, const u32* __restrict__ p
   dst.id[j] = id;
 }
 
+// plain float4 copy (bandwidth ceiling / counter calibration)
+__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, u64 n) {
+  u64 i = (u64)blockIdx.x * 256u + threadIdx.x;
+  u64 stride = (u64)gridDim.x * 256u;
+  for (; i < n; i += stride) dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------- host side
 static inline u32 cdiv(u64 a, u64 b) { return (u32)((a + b - 1) / b); }
 
@@ -267,6 +274,13 @@ void launch_reorder(Sim* s) {
   s->cur = dst;
   s->vcur ^= 1;
   s->prof_mark("reorder");
+}
+
+void launch_copy(Sim* s, const float4* src, float4* dst, u64 n) {
+  u32 blocks = (u32)((n + 255) / 256);
+  if (blocks > 8192u) blocks = 8192u;
+  if (blocks == 0) blocks = 1;
+  hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, s->stream, src, dst, n);
 }
 
 }  // namespace xpbi

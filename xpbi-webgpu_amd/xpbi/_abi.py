@@ -75,6 +75,7 @@ SIGNATURES = {
     "xpbi_time_substeps_detail": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float),
                                             C.POINTER(C.c_float), C.POINTER(C.c_int), C.c_int,
                                             C.POINTER(C.c_int)]),
+    "xpbi_copy_bandwidth": (C.c_int, [_P, C.c_size_t, C.c_int, C.POINTER(C.c_float)]),
     "xpbi_set_residual_monitor": (C.c_int, [_P, C.c_int]),
     "xpbi_read_residuals": (C.c_int, [_P, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_int)]),
 }

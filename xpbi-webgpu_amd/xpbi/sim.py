@@ -166,6 +166,12 @@ class Simulation:
             out[lib().xpbi_profile_label(self._h, k).decode()] = (ms[k], cnt[k])
         return tot.value, out
 
+    def copy_bandwidth(self, nbytes, reps=3):
+        """Milliseconds of one device-to-device float4 copy of nbytes (the stream-copy ceiling)."""
+        ms = C.c_float()
+        check(lib().xpbi_copy_bandwidth(self._h, int(nbytes), int(reps), C.byref(ms)), self._h, "copy_bandwidth")
+        return ms.value
+
     def sync(self):
         check(lib().xpbi_sync(self._h), self._h, "sync")
 

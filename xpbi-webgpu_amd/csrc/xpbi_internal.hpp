@@ -4,7 +4,7 @@
 #include <string>
 #include <vector>
 #include "xpbi_dev.hpp"
-#include "../../include/xpbi.h"
+#include "xpbi.h"
 
 namespace xpbi {
 

@@ -70,6 +70,59 @@ __device__ __forceinline__ void forSupport(const KP& P, const u32* __restrict__ 
   }
 }
 
+#ifndef XPBI_GATHER_BATCH
+#define XPBI_GATHER_BATCH 8
+#endif
+constexpr int kBatch = XPBI_GATHER_BATCH;
+
+// As forSupport, with the neighbour gathers issued in batches of kBatch
+// independent loads (and the next batch's indices prefetched) so that several
+// cache misses are in flight per lane: load(b) gathers the per-neighbour data,
+// use(b, pos[b], dxv, |dxv|^2, data) consumes it, in list order.
+template <class Load, class Use>
+__device__ __forceinline__ void forSupportG(const KP& P, const u32* __restrict__ cs, const float4* __restrict__ pos,
+                                            const NLc& nl, u32 i, f3 xi, Load&& load, Use&& use) {
+  const u32 n = nl.cnt[i];
+  if (n <= nl.K) {
+    const u32* __restrict__ p = nl.idx + i;
+    u32 bn[kBatch];
+#pragma unroll
+    for (int j = 0; j < kBatch; j++) bn[j] = ((u32)j < n) ? p[(size_t)j * nl.N] : i;
+    for (u32 k = 0; k < n; k += kBatch) {
+      u32 b[kBatch];
+      float4 pb[kBatch];
+      decltype(load(0u)) d[kBatch];
+#pragma unroll
+      for (int j = 0; j < kBatch; j++) {
+        b[j] = bn[j];
+        pb[j] = pos[b[j]];
+        d[j] = load(b[j]);
+      }
+#pragma unroll
+      for (int j = 0; j < kBatch; j++) {
+        u32 kk = k + kBatch + (u32)j;
+        bn[j] = (kk < n) ? p[(size_t)kk * nl.N] : i;
+      }
+#pragma unroll
+      for (int j = 0; j < kBatch; j++) {
+        if (k + (u32)j < n) {
+          const f3 dd = xi - xyz(pb[j]);
+          use(b[j], pb[j], dd, dot(dd, dd), d[j]);
+        }
+      }
+    }
+  } else {
+    forRows(P, cs, cellCoord(P, xi), [&](u32 b) {
+      if (b != i) {
+        const float4 pb = pos[b];
+        const f3 d = xi - xyz(pb);
+        const float r2 = dot(d, d);
+        if (r2 < P.supp2) use(b, pb, d, r2, load(b));
+      }
+    });
+  }
+}
+
 // material tables staged in LDS at kernel entry
 struct MatLds {
   DevMat m[8];
@@ -133,20 +186,48 @@ __global__ __launch_bounds__(256) void k_computeL(KP P, const float4* __restrict
 }
 
 // ---------------------------------------------------------------- velocity gradient
-// simkernels.js:224-245
-__device__ __forceinline__ f33 velocityGradient(const KP& P, const u32* __restrict__ cs,
-                                                const float4* __restrict__ pos, const float4* __restrict__ vin,
-                                                const NLc& nl, u32 solidMask, u32 i, f3 xi, f3 vi, const f33& Li) {
-  f33 G = mzero<float>();
-  forSupport(P, cs, pos, nl, i, xi, [&](u32 b, float4 pb, f3 dxv, float r2) {
-    if (solidTag(solidMask, pb.w)) {
-      f3 gt = mul(Li, gradWfast(P, dxv, r2));
-      float4 vb = vin[b];
-      f3 dv = (xyz(vb) - vi) * vb.w;
-      G = madd(G, outer(dv, gt));
-    }
-  });
-  return G;
+// simkernels.js:224-245.  With g_b = gradW(x_i - x_b) and w_b = V_b (v_b - v_i),
+//   grad v = sum_b w_b (x) (L_i g_b) = (sum_b w_b (x) g_b) L_i^T = M L_i^T,
+// so the sweep accumulates the first moment M and L_i is applied once.
+struct Moments {
+  f33 M;    // sum_b V_b (v_b - v_i) (x) g_b
+  f3 s;     // sum_b V_b g_b
+  float q00, q11, q22, q01, q02, q12;  // sum_b (V_b^2 / m_b) g_b g_b^T
+};
+template <bool kWithQ>
+__device__ __forceinline__ Moments neighbourMoments(const KP& P, const u32* __restrict__ cs,
+                                                    const float4* __restrict__ pos, const float4* __restrict__ vin,
+                                                    const NLc& nl, const float* invMass, u32 solidMask, u32 i,
+                                                    f3 xi, f3 vi) {
+  Moments m;
+  m.M = mzero<float>();
+  m.s = splat3(0.0f);
+  m.q00 = m.q11 = m.q22 = m.q01 = m.q02 = m.q12 = 0.0f;
+  forSupportG(P, cs, pos, nl, i, xi, [&](u32 b) { return vin[b]; },
+              [&](u32 b, float4 pb, f3 dxv, float r2, float4 vb) {
+                if (solidTag(solidMask, pb.w)) {
+                  const f3 g = gradWfast(P, dxv, r2);
+                  const f3 w = (xyz(vb) - vi) * vb.w;
+                  m.M = madd(m.M, outer(w, g));
+                  if (kWithQ) {
+                    m.s = m.s + g * vb.w;
+                    const float c = vb.w * vb.w * invMass[matOfTag(pb.w)];
+                    const f3 cg = g * c;
+                    m.q00 += cg.x * g.x;
+                    m.q11 += cg.y * g.y;
+                    m.q22 += cg.z * g.z;
+                    m.q01 += cg.x * g.y;
+                    m.q02 += cg.x * g.z;
+                    m.q12 += cg.y * g.z;
+                  }
+                }
+              });
+  return m;
+}
+
+// Frobenius product A : B of two column-major matrices
+__device__ __forceinline__ float frob(const f33& A, const f33& B) {
+  return dot(A.c[0], B.c[0]) + dot(A.c[1], B.c[1]) + dot(A.c[2], B.c[2]);
 }
 
 __device__ __forceinline__ void blockReduce2(double a, double b, double* __restrict__ out) {
@@ -188,7 +269,8 @@ __global__ __launch_bounds__(256) void k_solveA(KP P, const DevMat* __restrict__
       const f33 Li = ldm(Lm, i);
       const float4 vi4 = vin[i];
       const f3 vi = xyz(vi4);
-      f33 gradV = velocityGradient(P, cs, pos, vin, nl, T.solidMask, i, xi, vi, Li);
+      const Moments mo = neighbourMoments<true>(P, cs, pos, vin, nl, T.invMass, T.solidMask, i, xi, vi);
+      const f33 gradV = mul(mo.M, transpose(Li));
       f33 Ftr = mul(madd(mident<float>(), mscale(gradV, P.dt)), Fn);
       SVD3<float> sv = svd3_guarded(Ftr);
       float st;
@@ -199,19 +281,15 @@ __global__ __launch_bounds__(256) void k_solveA(KP P, const DevMat* __restrict__
         f3 dpsi = stvkDPsi(sig, m.mu, m.lam);
         f33 Pk = diagMul(sv.U, dpsi / C, sv.V);
         f33 GL = mul(mul(Pk, transpose(Fn)), Li);
-        float denom = 0.0f, cdot = 0.0f;
-        f3 sgrad = splat3(0.0f);
-        forSupport(P, cs, pos, nl, i, xi, [&](u32 b, float4 pb, f3 dxv, float r2) {
-          if (solidTag(T.solidMask, pb.w)) {
-            float4 vb = vin[b];
-            f3 gb = vb.w * mul(GL, gradWfast(P, dxv, r2));
-            denom = denom + dot(gb, gb) * T.invMass[matOfTag(pb.w)];
-            sgrad = sgrad - gb;
-            cdot = cdot + dot(gb, xyz(vb));
-          }
-        });
+        // the reference's second sweep (simkernels.js:284-299) from the moments:
+        //   g'_b = V_b GL g_b, sgrad = -sum g'_b = -GL s,
+        //   sum |g'_b|^2 / m_b = (GL^T GL) : Q,  cdot = sum g'_b.(v_b - v_i) = GL : M
+        const f3 sgrad = -mul(GL, mo.s);
+        const float h00 = dot(GL.c[0], GL.c[0]), h11 = dot(GL.c[1], GL.c[1]), h22 = dot(GL.c[2], GL.c[2]);
+        const float h01 = dot(GL.c[0], GL.c[1]), h02 = dot(GL.c[0], GL.c[2]), h12 = dot(GL.c[1], GL.c[2]);
+        float denom = h00 * mo.q00 + h11 * mo.q11 + h22 * mo.q22 + 2.0f * (h01 * mo.q01 + h02 * mo.q02 + h12 * mo.q12);
         denom = denom + dot(sgrad, sgrad) / m.mass;
-        cdot = cdot + dot(sgrad, vi);
+        const float cdot = frob(GL, mo.M);
         const float g = P.kDamp;
         const float num = -C - P.alphaT * a.y - g * P.dt * cdot;
         const float dlam = P.omega * num / ((1.0f + g) * denom + P.alphaT);
@@ -280,6 +358,11 @@ __device__ __forceinline__ f3 applyBoundary(const KP& P, f3 xcIn, f3 vIn, f3 xn,
   return vv;
 }
 
+struct SpGather {
+  float4 v, a, b;
+  float c;
+};
+
 // simkernels.js:315-366
 __global__ __launch_bounds__(256) void k_solveB(KP P, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
                                                 const float4* __restrict__ vin, M3C Sp,
@@ -301,12 +384,14 @@ __global__ __launch_bounds__(256) void k_solveB(KP P, const DevMat* __restrict__
     const float Vi = vi4.w;
     const f3 xci = xi + P.dt * xyz(vi4);
     const float minD = P.minD;
-    forSupport(P, cs, pos, nl, i, xi, [&](u32 b, float4 pb4, f3 dxv, float r2) {
+    forSupportG(P, cs, pos, nl, i, xi, [&](u32 b) { return SpGather{vin[b], Sp.a[b], Sp.b[b], Sp.c[b]}; },
+                [&](u32 b, float4 pb4, f3 dxv, float r2, const SpGather& g) {
       if (solidTag(T.solidMask, pb4.w)) {
         f3 pb = xyz(pb4);
         f3 dxp = -dxv;  // x_b - x_i
-        dv = dv + Vi * mul(ldm(Sp, b), gradWfast(P, dxp, r2));
-        f3 xcb = pb + P.dt * xyz(vin[b]);
+        const f33 Spb{{f3{g.a.x, g.a.y, g.a.z}, f3{g.a.w, g.b.x, g.b.y}, f3{g.b.z, g.b.w, g.c}}};
+        dv = dv + Vi * mul(Spb, gradWfast(P, dxp, r2));
+        f3 xcb = pb + P.dt * xyz(g.v);
         f3 dxc = xci - xcb;
         float d2 = dot(dxc, dxc);
         if (d2 < minD * minD && d2 > 1e-18f) {
@@ -419,12 +504,10 @@ __global__ __launch_bounds__(256) void k_xsph(KP P, const DevMat* __restrict__ m
   const f3 vi = xyz(vi4);
   const bool iSolid = solidTag(T.solidMask, pi.w);
   f3 acc = splat3(0.0f);
-  forSupport(P, cs, pos, nl, i, xi, [&](u32 b, float4 pb, f3 dxv, float r2) {
-    if (solidTag(T.solidMask, pb.w) == iSolid) {
-      float4 vb = vin[b];
-      acc = acc + vb.w * (xyz(vb) - vi) * kernelWfast(P, r2);
-    }
-  });
+  forSupportG(P, cs, pos, nl, i, xi, [&](u32 b) { return vin[b]; },
+              [&](u32 b, float4 pb, f3 dxv, float r2, float4 vb) {
+                if (solidTag(T.solidMask, pb.w) == iSolid) acc = acc + vb.w * (xyz(vb) - vi) * kernelWfast(P, r2);
+              });
   float c = P.xsphC;
   if (!iSolid) c = P.xsphC * 10.0f;
   f3 v = vi + c * acc;
@@ -447,7 +530,8 @@ __global__ __launch_bounds__(256) void k_updateF(KP P, const DevMat* __restrict_
   MatV m = hardenedMat(matv(T, mi), state);
   const f3 xi = xyz(pi);
   const f33 Li = ldm(Lm, i);
-  f33 gradV = velocityGradient(P, cs, pos, vin, nl, T.solidMask, i, xi, xyz(vin[i]), Li);
+  const Moments mo = neighbourMoments<false>(P, cs, pos, vin, nl, T.invMass, T.solidMask, i, xi, xyz(vin[i]));
+  const f33 gradV = mul(mo.M, transpose(Li));
   f33 Ftr = mul(madd(mident<float>(), mscale(gradV, P.dt)), ldm(Fm, i));
   SVD3<float> sv = svd3_guarded(Ftr);
   float st;

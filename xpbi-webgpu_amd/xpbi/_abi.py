@@ -8,7 +8,8 @@ import ctypes as C
 import os
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PKG_DIR, "libxpbi.so")
+# XPBI_LIB points at another build of the library (e.g. an instrumented one)
+LIB_PATH = os.environ.get("XPBI_LIB") or os.path.join(PKG_DIR, "libxpbi.so")
 
 OK, ERR_ARG, ERR_HIP, ERR_STATE, ERR_NOMEM = 0, -1, -2, -3, -4
 

@@ -42,19 +42,21 @@ def _rows(a, b, floor):
     return d / np.maximum(np.linalg.norm(np.asarray(b, np.float64), axis=1), floor)
 
 
-def check_field(name, gpu, o32, o64, floor, tol, label):
+def check_field(name, gpu, o32s, o64, floor, tol, label):
     """GPU vs the f32 oracle (rel per particle) with the f32 conditioning floor taken into account.
 
-    * the GPU is no farther from the f64 oracle than 3x the f32 oracle is (or tol);
-    * when the f32 oracle is within tol of the f64 one everywhere, every particle of the GPU
-      is within tol of the f32 oracle;
+    o32s holds the f32 oracle on the snapshot and on 1-ulp perturbations of it; per
+    particle, the f32 spread is the largest distance of any of them to the f64 oracle.
+
+    * the GPU is no farther from the f64 oracle than 3x the f32 spread (or tol);
+    * when the f32 spread is within tol everywhere, every particle of the GPU is within
+      tol of the f32 oracle;
     * otherwise (ill-conditioned particles, where 1 ulp of input moves any f32 result past
-      tol) the GPU has at most 3x as many particles past tol as the f32 oracle has against
-      the f64 oracle.
+      tol) the GPU has at most 3x as many particles past tol as the f32 spread has.
     """
-    e = _rows(gpu, o32, floor)
+    e = _rows(gpu, o32s[0], floor)
     e64 = _rows(gpu, o64, floor)
-    n64 = _rows(o32, o64, floor)
+    n64 = np.max([_rows(o, o64, floor) for o in o32s], axis=0)
     bad, bad_o = int((e > tol).sum()), int((n64 > tol).sum())
     print(f"  {name}: gpu-f32 max {e.max(initial=0):.2e} (>tol: {bad}) | gpu-f64 max {e64.max(initial=0):.2e} "
           f"| f32-f64 max {n64.max(initial=0):.2e} (>tol: {bad_o})", flush=True)
@@ -66,14 +68,14 @@ def check_field(name, gpu, o32, o64, floor, tol, label):
     return float(e.max(initial=0))
 
 
-def check_vel(vel, o32, o64, vs, label):
+def check_vel(vel, o32s, o64, vs, label):
     """Velocities relative to the largest speed: near-zero-strain particles make single
     velocities ill-conditioned in f32, so the bar is statistical: the GPU's 99.9th
-    percentile distance to the f64 oracle is within 3x the f32 oracle's (or 1e-4), and its
-    largest distance within 10x the f32 oracle's largest (or 1e-3)."""
+    percentile distance to the f64 oracle is within 3x the f32 spread's (or 1e-4), and its
+    largest distance within 10x the f32 spread's largest (or 1e-3).  o32s as in check_field."""
     e64 = np.abs(vel - o64).max(axis=1) / vs
-    n64 = np.abs(o32 - o64).max(axis=1) / vs
-    e = np.abs(vel - o32).max(axis=1) / vs
+    n64 = np.max([np.abs(o - o64).max(axis=1) / vs for o in o32s], axis=0)
+    e = np.abs(vel - o32s[0]).max(axis=1) / vs
     q, qn = float(np.quantile(e64, 0.999)), float(np.quantile(n64, 0.999))
     print(f"  vel: gpu-f32 max {e.max():.2e} | gpu-f64 q999 {q:.2e} max {e64.max():.2e} | "
           f"f32-f64 q999 {qn:.2e} max {n64.max():.2e}", flush=True)
@@ -82,15 +84,29 @@ def check_vel(vel, o32, o64, vs, label):
     return float(e.max())
 
 
-def compare(sim, ref, ref64, label, vel_scale=None):
+def compare(sim, refs, ref64, label, vel_scale=None):
+    """refs: the f32 oracle on the snapshot, then on its 1-ulp perturbations."""
+    ref = refs[0]
     pos, vel, F = sim.read("POS"), sim.read("VEL"), sim.read("F")
     assert np.array_equal(sim.read("ID"), ref.id), f"{label}: id channel differs"
     print(f"{label}: N={sim.N}", flush=True)
-    ep = check_field("pos", pos[:, :3], ref.pos[:, :3], ref64.pos[:, :3], 1e-6, POS_TOL, label)
-    eF = check_field("F", F3(F), F3(ref.F), F3(ref64.F), 1.0, F_TOL, label)
+    ep = check_field("pos", pos[:, :3], [r.pos[:, :3] for r in refs], ref64.pos[:, :3], 1e-6, POS_TOL, label)
+    eF = check_field("F", F3(F), [F3(r.F) for r in refs], F3(ref64.F), 1.0, F_TOL, label)
     vs = vel_scale or max(float(np.abs(ref.velA[:, :3]).max()), 1e-3)
-    ev = check_vel(vel[:, :3].astype(np.float64), ref.velA[:, :3].astype(np.float64), ref64.velA[:, :3], vs, label)
+    ev = check_vel(vel[:, :3].astype(np.float64), [r.velA[:, :3].astype(np.float64) for r in refs],
+                   ref64.velA[:, :3], vs, label)
     return ep, eF, ev
+
+
+def perturbed(st, seed):
+    """The snapshot with velocity and F moved by one ulp (random sign per entry); positions,
+    and so the sort, are unchanged."""
+    rng = np.random.default_rng(seed)
+    p = st.copy()
+    for a in (p.velA, p.F):
+        step = rng.choice(np.array([-1, 1], np.int8), size=a.shape)
+        a[:] = np.where(step > 0, np.nextafter(a, np.float32(np.inf)), np.nextafter(a, np.float32(-np.inf)))
+    return p
 
 
 def deformed(F12, aux, tags, scene):
@@ -106,14 +122,15 @@ def one_substep_parity(sim, scene, oracle, label, iters=None, water=None):
     water = scene["hasWater"] if water is None else water
     st32 = snapshot(sim, oracle)
     st64 = st32.copy(np.float64)
+    variants = [st32] + [perturbed(st32, k) for k in (1, 2)]
     mats = mats_record(scene)
     sim.substep(iters, water)
-    oracle.substep(st32, sim.params, mats, iters, water)
-    oracle.substep(st64, sim.params, mats, iters, water)
+    for st in variants + [st64]:
+        oracle.substep(st, sim.params, mats, iters, water)
     assert np.array_equal(sim.read("PERM"), st32.sortedIdx), f"{label}: permutation differs"
     assert np.array_equal(sim.read("CELLSTART"), st32.cellTab), f"{label}: cell starts differ"
     assert deformed(st32.F, st32.aux, st32.pos[:, 3], scene), f"{label}: snapshot is undeformed"
-    return compare(sim, st32, st64, label)
+    return compare(sim, variants, st64, label)
 
 
 def test_sort_bit_exact_c2(oracle):
@@ -179,7 +196,7 @@ def test_staged_parity_sand(oracle):
             vs = max(float(np.abs(st.velA[:, :3]).max()), 1e-3)
             eF = rel_err_rows(F3(sim.read("F")), F3(st.F), floor=1.0)
             print(f"{stage}: F {eF:.2e}", flush=True)
-            check_vel(sim.read("VEL")[:, :3].astype(np.float64), st.velA[:, :3].astype(np.float64),
+            check_vel(sim.read("VEL")[:, :3].astype(np.float64), [st.velA[:, :3].astype(np.float64)],
                       st64.velA[:, :3], vs, stage)
             assert eF < F_TOL
     sim.close()

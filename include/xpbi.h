@@ -198,6 +198,20 @@ int xpbi_copy_bandwidth(xpbi_sim* sim, size_t bytes, int reps, float* ms);
 int xpbi_set_residual_monitor(xpbi_sim* sim, int enabled);
 int xpbi_read_residuals(xpbi_sim* sim, double* out, int cap, int* count);
 
+/* Residual-driven Jacobi loop (replaces the fixed count of web/js/sim.js:256;
+ * no reference counterpart).  With tol > 0, the `iterations` argument of
+ * xpbi_substep / xpbi_step is a cap: iteration k computes
+ * r_k = ||C + alpha~ lambda + gamma dt dC.v||_2 / ||C at iteration 0||_2 over solid
+ * particles (the numerator of Delta-lambda / omega, simkernels.js:305) and is the
+ * last one applied once r_k <= tol; the test runs on the device, so a substep
+ * needs no host synchronisation.  check_every > 0 also lets the host stop
+ * enqueuing iterations: it polls the device flag every check_every iterations
+ * (one stream synchronisation each).  tol = 0 restores the fixed count. */
+int xpbi_set_tolerance(xpbi_sim* sim, float tol, int check_every);
+/* Jacobi iterations applied by the last substep and, with tol > 0, its final
+ * residual r_k (-1 otherwise).  Synchronises. */
+int xpbi_last_iterations(xpbi_sim* sim, int* iterations, double* residual);
+
 #ifdef __cplusplus
 }
 #endif

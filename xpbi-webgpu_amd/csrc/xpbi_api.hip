@@ -40,6 +40,8 @@ void free_all(Sim* s) {
   if (s->stream) hipStreamSynchronize(s->stream);
   for (void* p : s->allocs) hipFree(p);
   s->allocs.clear();
+  if (s->ctlHost) hipHostFree(s->ctlHost);
+  s->ctlHost = nullptr;
   for (hipEvent_t e : s->events) hipEventDestroy(e);
   s->events.clear();
   if (s->stream) hipStreamDestroy(s->stream);
@@ -233,6 +235,8 @@ int xpbi_create(const xpbi_scene_desc* d, xpbi_sim** out) {
   A(dalloc(s, &s->dMats, 8));
   A(dalloc(s, &s->residPart, 2ull * (gN > 0 ? gN : 1)));
   A(dalloc(s, &s->residOut, 2ull * kMaxResidIters));
+  A(dalloc(s, &s->ctl, 1));
+  A(hipHostMalloc((void**)&s->ctlHost, sizeof(SolveCtl), hipHostMallocDefault));
   if (!ok) {
     free_all(s);
     delete h;
@@ -580,11 +584,44 @@ int xpbi_read_residuals(xpbi_sim* h, double* out, int cap, int* count) {
   int r = xpbi_sync(h);
   if (r) return r;
   int n = s->residIters;
+  if (s->tol > 0.0f && n > 0) {
+    // iterations past the stopping one left no entry
+    SolveCtl c;
+    if ((r = d2h(s, &c, s->ctl, sizeof(c)))) return r;
+    if ((int)c.iters < n) n = (int)c.iters;
+  }
   if (count) *count = n;
   int m = n < cap ? n : cap;
   if (m > 0) {
     if (!out) return fail(s, XPBI_ERR_ARG, "out is null");
     return d2h(s, out, s->residOut, 2ull * m * sizeof(double));
+  }
+  return XPBI_OK;
+}
+
+int xpbi_set_tolerance(xpbi_sim* h, float tol, int check_every) {
+  if (!h) return XPBI_ERR_ARG;
+  Sim* s = &h->s;
+  if (!(tol >= 0.0f) || check_every < 0) return fail(s, XPBI_ERR_ARG, "tol must be >= 0 and check_every >= 0");
+  s->tol = tol;
+  s->checkEvery = check_every;
+  return XPBI_OK;
+}
+
+int xpbi_last_iterations(xpbi_sim* h, int* iterations, double* residual) {
+  if (!h) return XPBI_ERR_ARG;
+  Sim* s = &h->s;
+  if (!iterations) return fail(s, XPBI_ERR_ARG, "iterations is null");
+  int r = xpbi_sync(h);
+  if (r) return r;
+  if (s->tol > 0.0f) {
+    SolveCtl c;
+    if ((r = d2h(s, &c, s->ctl, sizeof(c)))) return r;
+    *iterations = (int)c.iters;
+    if (residual) *residual = c.c0sq > 0.0 ? sqrt(c.r2 / c.c0sq) : 0.0;
+  } else {
+    *iterations = s->lastIters;
+    if (residual) *residual = -1.0;
   }
   return XPBI_OK;
 }

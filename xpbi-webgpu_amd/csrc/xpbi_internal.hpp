@@ -29,6 +29,25 @@ struct PSetC {
   const u32* id;
 };
 
+// device-side state of the residual-driven Jacobi loop.  The iteration-k
+// residual is r_k^2 = sum num^2 / sum C0^2 over solid particles, where num is
+// the numerator of Delta-lambda / omega at the start of iteration k and C0 the
+// constraints of iteration 0.  Once r_k <= tol, iteration k is the last one
+// applied: later launches of the loop exit at once.
+struct SolveCtl {
+  u32 done;
+  u32 iters;  // Jacobi iterations applied
+  u32 pad[2];
+  double c0sq;
+  double r2;  // last residual numerator^2
+};
+
+// iteration k of the loop is skipped once an earlier iteration was the last:
+// the stopping iteration itself (iters == k + 1) still runs its solveB / PBF
+__device__ __forceinline__ bool skipIteration(const SolveCtl* ctl, u32 k) {
+  return ctl && ctl->done && ctl->iters <= k;
+}
+
 constexpr int kMaxProfileMarks = 4096;
 constexpr int kMaxResidIters = 4096;
 
@@ -72,6 +91,13 @@ struct Sim {
   double* residPart = nullptr;  // 2 * blocks
   double* residOut = nullptr;   // 2 * kMaxResidIters
   int residIters = 0;
+
+  // residual-driven loop (tol > 0): `iterations` becomes a cap
+  float tol = 0.0f;
+  int checkEvery = 4;       // host polls the done flag every checkEvery iterations (0: never)
+  SolveCtl* ctl = nullptr;  // device
+  SolveCtl* ctlHost = nullptr;  // pinned mirror for polling
+  int lastIters = 0;        // fixed-iteration count of the last substep (tol == 0)
 
   // profiling
   bool profiling = false;

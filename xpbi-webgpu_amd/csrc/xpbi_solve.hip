@@ -53,7 +53,9 @@ __global__ __launch_bounds__(256) void k_computeL(KP P, const float4* __restrict
 __global__ __launch_bounds__(256) void k_solveA(KP P, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
                                                 const float4* __restrict__ vin, M3C Fm, M3C Lm,
                                                 float4* __restrict__ aux, M3P Sp, float4* __restrict__ selfDv,
-                                                const u32* __restrict__ cs, NLc nl, double* __restrict__ part) {
+                                                const u32* __restrict__ cs, NLc nl, double* __restrict__ part,
+                                                const SolveCtl* __restrict__ ctl, u32 k) {
+  if (skipIteration(ctl, k)) return;
   __shared__ MatLds T;
   loadMatLds(T, mats);
   const u32 i = xcdBlock() * 256u + threadIdx.x;
@@ -69,21 +71,60 @@ __global__ __launch_bounds__(256) void k_solveA(KP P, const DevMat* __restrict__
   if (part) blockReduce2((double)r2, (double)c2, part + 2 * blockIdx.x);
 }
 
-// deterministic second stage of the residual reduction
-__global__ __launch_bounds__(256) void k_resid_final(const double* __restrict__ part, u32 nb, double* __restrict__ out) {
+// deterministic second stage of the residual reduction; with ctl, also the
+// stopping test of iteration k (skipped once the loop is done)
+__global__ __launch_bounds__(256) void k_resid_final(const double* __restrict__ part, u32 nb, double* __restrict__ out,
+                                                     SolveCtl* __restrict__ ctl, u32 k, double tol2) {
+  if (skipIteration(ctl, k)) return;
+  __shared__ double s_out[2];
   double a = 0.0, b = 0.0;
-  for (u32 k = threadIdx.x; k < nb; k += 256u) {
-    a += part[2 * k];
-    b += part[2 * k + 1];
+  for (u32 j = threadIdx.x; j < nb; j += 256u) {
+    a += part[2 * j];
+    b += part[2 * j + 1];
   }
-  blockReduce2(a, b, out);
+  blockReduce2(a, b, s_out);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (out) {
+      out[0] = s_out[0];
+      out[1] = s_out[1];
+    }
+    if (ctl) {
+      if (k == 0) ctl->c0sq = s_out[1];
+      const double c0 = (k == 0) ? s_out[1] : ctl->c0sq;
+      ctl->iters = k + 1;
+      ctl->r2 = s_out[0];
+      if (c0 <= 0.0 || s_out[0] <= tol2 * c0) ctl->done = 1u;
+    }
+  }
+}
+
+__global__ void k_ctl_reset(SolveCtl* __restrict__ ctl) {
+  if (threadIdx.x == 0) {
+    ctl->done = 0u;
+    ctl->iters = 0u;
+    ctl->c0sq = 0.0;
+    ctl->r2 = 0.0;
+  }
+}
+
+// after a loop that stopped early the velocity ping-pong has moved `iters`
+// times instead of `launched`: when the two differ in parity, the final
+// velocity is in `src` and is copied to `dst`, where the schedule expects it
+__global__ __launch_bounds__(256) void k_parity_fix(const SolveCtl* __restrict__ ctl, u32 launched, u32 n,
+                                                    const float4* __restrict__ src, float4* __restrict__ dst) {
+  if (((launched - ctl->iters) & 1u) == 0u) return;
+  const u32 i = xcdBlock() * 256u + threadIdx.x;
+  if (i < n) dst[i] = src[i];
 }
 
 // simkernels.js:315-366
 __global__ __launch_bounds__(256) void k_solveB(KP P, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
                                                 const float4* __restrict__ vin, M3C Sp,
                                                 const float4* __restrict__ selfDv, const u32* __restrict__ cs,
-                                                NLc nl, float4* __restrict__ vout) {
+                                                NLc nl, float4* __restrict__ vout, const SolveCtl* __restrict__ ctl,
+                                                u32 k) {
+  if (skipIteration(ctl, k)) return;
   __shared__ MatLds T;
   loadMatLds(T, mats);
   const u32 i = xcdBlock() * 256u + threadIdx.x;
@@ -100,7 +141,8 @@ __global__ __launch_bounds__(256) void k_solveB(KP P, const DevMat* __restrict__
 // simkernels.js:407-439
 __global__ __launch_bounds__(256) void k_pbfA(KP P, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
                                               const float4* __restrict__ vin, float4* __restrict__ aux,
-                                              const u32* __restrict__ cs) {
+                                              const u32* __restrict__ cs, const SolveCtl* __restrict__ ctl, u32 k) {
+  if (skipIteration(ctl, k)) return;
   __shared__ MatLds T;
   loadMatLds(T, mats);
   const u32 i = xcdBlock() * 256u + threadIdx.x;
@@ -135,7 +177,9 @@ __global__ __launch_bounds__(256) void k_pbfA(KP P, const DevMat* __restrict__ m
 // simkernels.js:441-477
 __global__ __launch_bounds__(256) void k_pbfB(KP P, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
                                               const float4* __restrict__ vin, const float4* __restrict__ aux,
-                                              const u32* __restrict__ cs, float4* __restrict__ vout) {
+                                              const u32* __restrict__ cs, float4* __restrict__ vout,
+                                              const SolveCtl* __restrict__ ctl, u32 k) {
+  if (skipIteration(ctl, k)) return;
   __shared__ MatLds T;
   loadMatLds(T, mats);
   const u32 i = xcdBlock() * 256u + threadIdx.x;
@@ -258,31 +302,51 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
   s->prof_mark("computeL");
   if (stop == XPBI_STAGE_COMPUTE_L) return;
   s->residIters = 0;
+  const bool loop = s->tol > 0.0f;
+  SolveCtl* ctl = loop ? s->ctl : nullptr;
+  const double tol2 = (double)s->tol * (double)s->tol;
+  if (loop) hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, st, ctl);
+  int launched = 0;
   for (int k = 0; k < iterations; k++) {
-    double* part = (s->residOn && k < kMaxResidIters) ? s->residPart : nullptr;
+    if (loop && s->checkEvery > 0 && k > 0 && k % s->checkEvery == 0) {
+      // stop enqueuing once the device has flagged convergence
+      hipMemcpyAsync(s->ctlHost, ctl, sizeof(SolveCtl), hipMemcpyDeviceToHost, st);
+      hipStreamSynchronize(st);
+      if (s->ctlHost->done) break;
+    }
+    double* out = (s->residOn && k < kMaxResidIters) ? s->residOut + 2 * k : nullptr;
+    double* part = (out || loop) ? s->residPart : nullptr;
     hipLaunchKernelGGL(k_solveA, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
                        (const float4*)s->vel[vi], cm(F), cm(s->L), S.aux, s->Sp, s->selfDv,
-                       (const u32*)s->cellStart, nl, part);
+                       (const u32*)s->cellStart, nl, part, (const SolveCtl*)ctl, (u32)k);
     s->prof_mark("solveA");
     if (part) {
-      hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(256), 0, st, (const double*)part, gN, s->residOut + 2 * k);
-      s->residIters = k + 1;
+      hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(256), 0, st, (const double*)part, gN, out, ctl, (u32)k, tol2);
+      if (out) s->residIters = k + 1;
     }
     hipLaunchKernelGGL(k_solveB, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
                        (const float4*)s->vel[vi], cm(s->Sp), (const float4*)s->selfDv, (const u32*)s->cellStart,
-                       nl, s->vel[vi ^ 1]);
+                       nl, s->vel[vi ^ 1], (const SolveCtl*)ctl, (u32)k);
     s->prof_mark("solveB");
     vi ^= 1;
     if (water) {
       hipLaunchKernelGGL(k_pbfA, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
-                         (const float4*)s->vel[vi], S.aux, (const u32*)s->cellStart);
+                         (const float4*)s->vel[vi], S.aux, (const u32*)s->cellStart, (const SolveCtl*)ctl, (u32)k);
       s->prof_mark("pbfA");
       hipLaunchKernelGGL(k_pbfB, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
-                         (const float4*)s->vel[vi], (const float4*)S.aux, (const u32*)s->cellStart, s->vel[vi ^ 1]);
+                         (const float4*)s->vel[vi], (const float4*)S.aux, (const u32*)s->cellStart, s->vel[vi ^ 1],
+                         (const SolveCtl*)ctl, (u32)k);
       s->prof_mark("pbfB");
       vi ^= 1;
     }
+    launched++;
   }
+  if (loop && !water && launched > 0) {
+    hipLaunchKernelGGL(k_parity_fix, dim3(gN), dim3(256), 0, st, (const SolveCtl*)ctl, (u32)launched, s->N,
+                       (const float4*)s->vel[vi ^ 1], s->vel[vi]);
+    s->prof_mark("parityFix");
+  }
+  s->lastIters = launched;
   if (stop == XPBI_STAGE_SOLVE) { s->vcur = vi; return; }
   hipLaunchKernelGGL(k_xsph, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
                      (const float4*)s->vel[vi], (const u32*)s->cellStart, nl, s->vel[vi ^ 1]);

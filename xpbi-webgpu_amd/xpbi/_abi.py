@@ -79,6 +79,8 @@ SIGNATURES = {
     "xpbi_copy_bandwidth": (C.c_int, [_P, C.c_size_t, C.c_int, C.POINTER(C.c_float)]),
     "xpbi_set_residual_monitor": (C.c_int, [_P, C.c_int]),
     "xpbi_read_residuals": (C.c_int, [_P, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_int)]),
+    "xpbi_set_tolerance": (C.c_int, [_P, C.c_float, C.c_int]),
+    "xpbi_last_iterations": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
 }
 
 _lib = None

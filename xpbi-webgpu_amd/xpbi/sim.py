@@ -208,6 +208,18 @@ class Simulation:
               self._h)
         return out[: 2 * min(n.value, cap)].reshape(-1, 2)
 
+    def set_tolerance(self, tol, check_every=4):
+        """Residual-driven Jacobi: with tol > 0 the iteration argument of substep/step is
+        a cap and each substep stops once ||C + alpha~ lambda + gamma dt dC.v|| /
+        ||C at iteration 0|| <= tol (tested on the device)."""
+        check(lib().xpbi_set_tolerance(self._h, float(tol), int(check_every)), self._h)
+
+    def last_iterations(self):
+        """(Jacobi iterations applied by the last substep, its final residual or -1)."""
+        n, r = C.c_int(), C.c_double()
+        check(lib().xpbi_last_iterations(self._h, C.byref(n), C.byref(r)), self._h)
+        return n.value, r.value
+
     def close(self):
         if getattr(self, "_h", None):
             lib().xpbi_destroy(self._h)

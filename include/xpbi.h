@@ -212,6 +212,60 @@ int xpbi_set_tolerance(xpbi_sim* sim, float tol, int check_every);
  * residual r_k (-1 otherwise).  Synchronises. */
 int xpbi_last_iterations(xpbi_sim* sim, int* iterations, double* residual);
 
+/* ---- Learned warm start (no reference counterpart: the reference starts every
+ * substep's multipliers at zero, simkernels.js:146, and bisects the HB flow
+ * rule over its full bracket, common.js:264-265).
+ *
+ * A 12 -> 64 -> 64 -> 2 MLP (BF16 weights and activations, f32 accumulation;
+ * canonical parameter order W1[64][12], b1[64], W2[64][64], b2[64], W3[2][64],
+ * b3[2], row = output unit) maps each solid particle's features at lambda = 0 to
+ * (rho, t): the warm start sets lambda0 = clamp(rho, +-8) d0, where d0 is the plain
+ * first step (-C - gamma dt dC.v) / ((1 + gamma) denom + alpha~), applies the
+ * matching velocity change through the constraint gather, and then runs plain
+ * Jacobi (to the tolerance of xpbi_set_tolerance, measured against the same
+ * ||C|| at lambda = 0 as a plain run).  t is the HB return map's predicted root
+ * position in its bisection bracket.  Features (x, 12 per particle):
+ * 100 log sigma of the trial F (3, descending), 100 tr log sigma and
+ * 100 |dev log sigma| after the return map, log10(mu) - 5, lambda / (lambda + 2 mu),
+ * kind / 6, material strength, support neighbours / 40,
+ * clamp(previous lambda / d0, +-8), alpha~ / ((1 + gamma) denom + alpha~). */
+#define XPBI_MLP_INPUTS 12
+#define XPBI_MLP_OUTPUTS 2
+#define XPBI_MLP_PARAMS 5122
+
+/* Enable the warm start with the given network (a copy is kept), or disable it
+ * with params = NULL. */
+int xpbi_set_warmstart(xpbi_sim* sim, const float* params, size_t n);
+/* Run one plain substep (like xpbi_substep) and return, per particle in the
+ * order of the substep, its features x (rows x 12) and training targets t
+ * (rows x 2): lambda at the end of the solve / d0 and the HB root position of
+ * the first iteration's return map (NaN where there is none).  rows = N. */
+int xpbi_collect_warmstart_pairs(xpbi_sim* sim, int iterations, int has_water, float* x, float* t, size_t rows);
+
+/* The MLP trainer: forward and backward on BF16 MFMA, Adam (beta 0.9, 0.999,
+ * eps 1e-8) on f32 master weights.  Loss = mean over rows of
+ * sum_k (y_k - t_k)^2, skipping NaN targets.  One trainer per device; calls on one
+ * trainer must not overlap. */
+typedef struct xpbi_mlp xpbi_mlp;
+int xpbi_mlp_create(int device, xpbi_mlp** out);
+void xpbi_mlp_destroy(xpbi_mlp* mlp);
+const char* xpbi_mlp_last_error(const xpbi_mlp* mlp);
+/* Set the f32 master weights (resets the Adam moments and step count). */
+int xpbi_mlp_set_params(xpbi_mlp* mlp, const float* params, size_t n);
+int xpbi_mlp_get_params(xpbi_mlp* mlp, float* params, size_t n);
+/* Training data: x rows x 12, t rows x 2 (copied to the device). */
+int xpbi_mlp_set_data(xpbi_mlp* mlp, const float* x, const float* t, size_t rows);
+/* y (rows x 2) = the network on x (rows x 12). */
+int xpbi_mlp_predict(xpbi_mlp* mlp, const float* x, size_t rows, float* y);
+/* Loss and its gradient (canonical order) over data rows [first, first + count). */
+int xpbi_mlp_loss_grad(xpbi_mlp* mlp, size_t first, size_t count, float* grad, float* loss);
+/* One Adam step with learning rate lr on grad (canonical order), or on the
+ * gradient of the last xpbi_mlp_loss_grad when grad = NULL. */
+int xpbi_mlp_adam_step(xpbi_mlp* mlp, const float* grad, float lr);
+/* Minibatch Adam over the data: each epoch visits the rows in a fresh order
+ * (seeded); epoch_loss (epochs floats, may be NULL) gets each epoch's mean loss. */
+int xpbi_mlp_train(xpbi_mlp* mlp, int epochs, int batch, float lr, uint64_t seed, float* epoch_loss);
+
 #ifdef __cplusplus
 }
 #endif

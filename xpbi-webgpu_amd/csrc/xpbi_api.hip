@@ -626,4 +626,63 @@ int xpbi_last_iterations(xpbi_sim* h, int* iterations, double* residual) {
   return XPBI_OK;
 }
 
+static int ensure_warmstart_buffers(Sim* s) {
+  if (s->wsFeat) return XPBI_OK;
+  hipSetDevice(s->dev);
+  bool ok = true;
+  auto A = [&](hipError_t r) { if (r != hipSuccess) ok = false; };
+  const u32 N = s->N;
+  A(dalloc(s, (uint8_t**)&s->wsPacked, mlp_packed_bytes()));
+  A(dalloc(s, &s->wsFeat, 16ull * N));
+  A(dalloc(s, &s->wsHb, N));
+  A(dalloc(s, &s->wsTarget, 2ull * N));
+  A(dalloc(s, &s->lamPrev, N));
+  if (!ok) {
+    s->wsFeat = nullptr;
+    return fail(s, XPBI_ERR_NOMEM, "warm-start buffers: device allocation failed");
+  }
+  return XPBI_OK;
+}
+
+int xpbi_set_warmstart(xpbi_sim* h, const float* params, size_t n) {
+  if (!h) return XPBI_ERR_ARG;
+  Sim* s = &h->s;
+  if (!params) {
+    s->wsMode = 0;
+    return XPBI_OK;
+  }
+  if (n != (size_t)XPBI_MLP_PARAMS) return fail(s, XPBI_ERR_ARG, "params must hold XPBI_MLP_PARAMS floats");
+  for (size_t k = 0; k < n; k++)
+    if (!std::isfinite(params[k])) return fail(s, XPBI_ERR_ARG, "non-finite MLP parameter");
+  int r = ensure_warmstart_buffers(s);
+  if (r) return r;
+  if ((r = xpbi_sync(h))) return r;
+  std::vector<uint8_t> packed(mlp_packed_bytes());
+  mlp_pack_host(params, packed.data());
+  if ((r = h2d(s, s->wsPacked, packed.data(), packed.size()))) return r;
+  s->wsMode = 1;
+  return XPBI_OK;
+}
+
+int xpbi_collect_warmstart_pairs(xpbi_sim* h, int iterations, int has_water, float* x, float* t, size_t rows) {
+  if (!h) return XPBI_ERR_ARG;
+  Sim* s = &h->s;
+  if (rows != s->N) return fail(s, XPBI_ERR_ARG, "rows must equal the particle count");
+  if (rows > 0 && (!x || !t)) return fail(s, XPBI_ERR_ARG, "x and t must not be null");
+  int r = ensure_warmstart_buffers(s);
+  if (r) return r;
+  const int mode = s->wsMode;
+  s->wsMode = 2;
+  r = run_substep(s, iterations, has_water, XPBI_STAGE_FULL);
+  s->wsMode = mode;
+  if (r) return r;
+  if ((r = xpbi_sync(h))) return r;
+  if (rows == 0) return XPBI_OK;
+  std::vector<float> f(16ull * rows);
+  if ((r = d2h(s, f.data(), s->wsFeat, f.size() * sizeof(float)))) return r;
+  if ((r = d2h(s, t, s->wsTarget, 2ull * rows * sizeof(float)))) return r;
+  for (size_t i = 0; i < rows; i++) memcpy(x + i * XPBI_MLP_INPUTS, &f[16 * i], XPBI_MLP_INPUTS * sizeof(float));
+  return XPBI_OK;
+}
+
 }  // extern "C"

@@ -404,8 +404,16 @@ __device__ __forceinline__ f3 rmVonMises(f3 sig, const MatV& m) {
   return vexp(en);
 }
 // common.js:244-280
-__device__ __forceinline__ f3 rmHerschelBulkley(f3 sig, const MatV& m, float dt) {
+//
+// hint in [0, 1] (learned warm start) is a predicted root position t in
+// s = sY + t (sPre - sY): the bisection then starts from [t - 1/64, t + 1/64]
+// when g changes sign across it (13 halvings reach the 2^-18 width of the
+// reference's 18 over the full bracket), else from the full bracket.  *tOut
+// receives the root position of the bisection (NaN when none ran).
+__device__ __forceinline__ f3 rmHerschelBulkley(f3 sig, const MatV& m, float dt, float hint = -1.0f,
+                                                float* tOut = nullptr) {
   float sigY = m.p0, hexp = m.p1, eta = m.p2;
+  if (tOut) *tOut = __builtin_nanf("");
   f3 s3 = safeSig(sig);
   float J = s3.x * s3.y * s3.z;
   f3 bbar = s3 * s3 * powf(J, -2.0f / 3.0f);
@@ -421,14 +429,25 @@ __device__ __forceinline__ f3 rmHerschelBulkley(f3 sig, const MatV& m, float dt)
   } else {
     float ih = 1.0f / hexp;
     float ei = powf(eta, ih);
+    auto g = [&](float x) { return ei * (x - sPre) + 2.0f * muT * dt * powf(wmax(x - sY, 0.0f), ih); };
     float lo = sY, hi = sPre;
+    int steps = 18;
+    if (hint >= 0.0f && hint <= 1.0f) {
+      const float a = wmax(hint - 1.0f / 64.0f, 0.0f), b = wmin(hint + 1.0f / 64.0f, 1.0f);
+      const float lo2 = sY + a * (sPre - sY), hi2 = sY + b * (sPre - sY);
+      if ((a == 0.0f || !(g(lo2) > 0.0f)) && (b == 1.0f || g(hi2) > 0.0f)) {
+        lo = lo2;
+        hi = hi2;
+        steps = 13;
+      }
+    }
 #pragma unroll 1
-    for (int it = 0; it < 18; it++) {
+    for (int it = 0; it < steps; it++) {
       float mid = 0.5f * (lo + hi);
-      float g = ei * (mid - sPre) + 2.0f * muT * dt * powf(wmax(mid - sY, 0.0f), ih);
-      if (g > 0.0f) hi = mid; else lo = mid;
+      if (g(mid) > 0.0f) hi = mid; else lo = mid;
     }
     sNew = 0.5f * (lo + hi);
+    if (tOut) *tOut = (sNew - sY) / (sPre - sY);
   }
   f3 bNew = devb * (sNew / sPre) + splat3(trb / 3.0f);
   bNew = vmaxs(bNew, 1e-6f);
@@ -518,13 +537,15 @@ __device__ __forceinline__ MatV hardenedMat(MatV m, float state) {
   return m;
 }
 // common.js:379-396
-__device__ __forceinline__ f3 returnMap(f3 sig, const MatV& m, float state, float dt, float* stateOut) {
+// hbHint / hbT: the HB bracket hint and root position (rmHerschelBulkley)
+__device__ __forceinline__ f3 returnMap(f3 sig, const MatV& m, float state, float dt, float* stateOut,
+                                        float hbHint = -1.0f, float* hbT = nullptr) {
   *stateOut = state;
   switch (m.kind) {
     case MAT_DP: return rmDruckerPrager(sig, m);
     case MAT_NACC: return rmNACC(sig, m, state, stateOut);
     case MAT_VM: return rmVonMises(sig, m);
-    case MAT_HB: return rmHerschelBulkley(sig, m, dt);
+    case MAT_HB: return rmHerschelBulkley(sig, m, dt, hbHint, hbT);
     case MAT_SNOW: {
       MatV mm = hardenedMat(m, state);
       f3 sNew = rmSnowClamp(sig, mm);

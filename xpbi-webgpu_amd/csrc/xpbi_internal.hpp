@@ -37,7 +37,8 @@ struct PSetC {
 struct SolveCtl {
   u32 done;
   u32 iters;  // Jacobi iterations applied
-  u32 pad[2];
+  u32 c0preset;  // c0sq was set by the warm-start features pass
+  u32 pad;
   double c0sq;
   double r2;  // last residual numerator^2
 };
@@ -99,6 +100,14 @@ struct Sim {
   SolveCtl* ctlHost = nullptr;  // pinned mirror for polling
   int lastIters = 0;        // fixed-iteration count of the last substep (tol == 0)
 
+  // learned warm start (mode 1) and training-pair collection (mode 2)
+  int wsMode = 0;
+  void* wsPacked = nullptr;  // packed MLP weights (device)
+  float* wsFeat = nullptr;   // N x 16 feature rows of the current substep
+  float* wsHb = nullptr;     // N: HB bracket hint (mode 1)
+  float* wsTarget = nullptr; // N x 2 training targets (mode 2)
+  float* lamPrev = nullptr;  // N: previous substep's final lambda, in the current order
+
   // profiling
   bool profiling = false;
   std::vector<hipEvent_t> events;
@@ -124,6 +133,9 @@ size_t radix_hist_needed(u64 n);
 int radix_passes(u32 C);
 void launch_solve_stages(Sim* s, int iterations, bool water, int stop);
 void launch_copy(Sim* s, const float4* src, float4* dst, u64 n);
+void launch_mlp_apply(Sim* s);
+void mlp_pack_host(const float* params, void* packed);
+size_t mlp_packed_bytes();
 
 }  // namespace xpbi
 

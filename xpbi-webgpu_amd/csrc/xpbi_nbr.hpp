@@ -217,49 +217,76 @@ __device__ __forceinline__ void blockReduce2(double a, double b, double* __restr
 }
 
 
-// simkernels.js:259-308 after the neighbour sweep: trial F, SVD, return map,
-// constraint, and the damped XPBD update from the moments.  Writes Sp, selfDv,
-// lambda; accumulates the residual numerator^2 and C^2 into *r2, *c2.
+// simkernels.js:259-300 after the neighbour sweep, for a solid particle: trial
+// F, SVD, return map, C = sqrt(2 Psi) and, when C >= 1e-7, GL, sgrad, the
+// Delta-lambda denominator and dC.v from the moments.
+struct Constraint {
+  float C, denom, cdot;
+  f33 GL;
+  f3 sgrad;
+  f3 sigTrial, sigPost;
+  float hbT;  // HB root position of the return map (NaN when no bisection ran)
+  bool active;
+};
+__device__ __forceinline__ Constraint evalConstraint(const KP& P, const MatLds& T, u32 i, float4 pi, float4 a,
+                                                     const Moments& mo, M3C Fm, M3C Lm, float hbHint,
+                                                     bool wantHbT) {
+  Constraint c;
+  const u32 mi = matOfTag(pi.w);
+  MatV m = hardenedMat(matv(T, mi), a.z);
+  const f33 Fn = ldm(Fm, i);
+  const f33 Li = ldm(Lm, i);
+  const f33 gradV = mul(mo.M, transpose(Li));
+  f33 Ftr = mul(madd(mident<float>(), mscale(gradV, P.dt)), Fn);
+  SVD3<float> sv = svd3_guarded(Ftr);
+  float st;
+  c.hbT = __builtin_nanf("");
+  f3 sig = returnMap(sv.sig, m, a.z, P.dt, &st, hbHint, wantHbT ? &c.hbT : nullptr);
+  c.sigTrial = sv.sig;
+  c.sigPost = sig;
+  float psi = stvkPsi(sig, m.mu, m.lam);
+  c.C = sqrtf(wmax(2.0f * psi, 0.0f));
+  c.active = c.C >= 1e-7f;
+  if (c.active) {
+    f3 dpsi = stvkDPsi(sig, m.mu, m.lam);
+    f33 Pk = diagMul(sv.U, dpsi / c.C, sv.V);
+    c.GL = mul(mul(Pk, transpose(Fn)), Li);
+    // the reference's second sweep (simkernels.js:284-299) from the moments:
+    //   g'_b = V_b GL g_b, sgrad = -sum g'_b = -GL s,
+    //   sum |g'_b|^2 / m_b = (GL^T GL) : Q,  cdot = sum g'_b.(v_b - v_i) = GL : M
+    const f33& GL = c.GL;
+    c.sgrad = -mul(GL, mo.s);
+    const float h00 = dot(GL.c[0], GL.c[0]), h11 = dot(GL.c[1], GL.c[1]), h22 = dot(GL.c[2], GL.c[2]);
+    const float h01 = dot(GL.c[0], GL.c[1]), h02 = dot(GL.c[0], GL.c[2]), h12 = dot(GL.c[1], GL.c[2]);
+    float denom = h00 * mo.q00 + h11 * mo.q11 + h22 * mo.q22 + 2.0f * (h01 * mo.q01 + h02 * mo.q02 + h12 * mo.q12);
+    c.denom = denom + dot(c.sgrad, c.sgrad) / m.mass;
+    c.cdot = frob(GL, mo.M);
+  }
+  return c;
+}
+
+// simkernels.js:259-308: the damped XPBD update of a solid particle from the
+// moments.  Writes Sp, selfDv, lambda; accumulates the residual numerator^2 and
+// C^2 into *r2, *c2.
 __device__ __forceinline__ void solveAFinish(const KP& P, const MatLds& T, u32 i, float4 pi, f3 vi, bool solid,
                                              const Moments& mo, M3C Fm, M3C Lm, float4* __restrict__ aux, M3P Sp,
-                                             float4* __restrict__ selfDv, float* r2, float* c2) {
+                                             float4* __restrict__ selfDv, float* r2, float* c2,
+                                             float hbHint = -1.0f) {
   bool wrote = false;
   if (solid) {
-    const u32 mi = matOfTag(pi.w);
     const float4 a = aux[i];
-    MatV m = hardenedMat(matv(T, mi), a.z);
-    const f33 Fn = ldm(Fm, i);
-    const f33 Li = ldm(Lm, i);
-    const f33 gradV = mul(mo.M, transpose(Li));
-    f33 Ftr = mul(madd(mident<float>(), mscale(gradV, P.dt)), Fn);
-    SVD3<float> sv = svd3_guarded(Ftr);
-    float st;
-    f3 sig = returnMap(sv.sig, m, a.z, P.dt, &st);
-    float psi = stvkPsi(sig, m.mu, m.lam);
-    float C = sqrtf(wmax(2.0f * psi, 0.0f));
-    if (C >= 1e-7f) {
-      f3 dpsi = stvkDPsi(sig, m.mu, m.lam);
-      f33 Pk = diagMul(sv.U, dpsi / C, sv.V);
-      f33 GL = mul(mul(Pk, transpose(Fn)), Li);
-      // the reference's second sweep (simkernels.js:284-299) from the moments:
-      //   g'_b = V_b GL g_b, sgrad = -sum g'_b = -GL s,
-      //   sum |g'_b|^2 / m_b = (GL^T GL) : Q,  cdot = sum g'_b.(v_b - v_i) = GL : M
-      const f3 sgrad = -mul(GL, mo.s);
-      const float h00 = dot(GL.c[0], GL.c[0]), h11 = dot(GL.c[1], GL.c[1]), h22 = dot(GL.c[2], GL.c[2]);
-      const float h01 = dot(GL.c[0], GL.c[1]), h02 = dot(GL.c[0], GL.c[2]), h12 = dot(GL.c[1], GL.c[2]);
-      float denom = h00 * mo.q00 + h11 * mo.q11 + h22 * mo.q22 + 2.0f * (h01 * mo.q01 + h02 * mo.q02 + h12 * mo.q12);
-      denom = denom + dot(sgrad, sgrad) / m.mass;
-      const float cdot = frob(GL, mo.M);
+    const Constraint c = evalConstraint(P, T, i, pi, a, mo, Fm, Lm, hbHint, false);
+    if (c.active) {
       const float g = P.kDamp;
-      const float num = -C - P.alphaT * a.y - g * P.dt * cdot;
-      const float dlam = P.omega * num / ((1.0f + g) * denom + P.alphaT);
+      const float num = -c.C - P.alphaT * a.y - g * P.dt * c.cdot;
+      const float dlam = P.omega * num / ((1.0f + g) * c.denom + P.alphaT);
       aux[i].y = a.y + dlam;
-      stm(Sp, i, mscale(GL, dlam));
-      f3 sd = sgrad * dlam;
+      stm(Sp, i, mscale(c.GL, dlam));
+      f3 sd = c.sgrad * dlam;
       selfDv[i] = make_float4(sd.x, sd.y, sd.z, 0.0f);
       wrote = true;
       *r2 += num * num;
-      *c2 += C * C;
+      *c2 += c.C * c.C;
     }
   }
   if (!wrote) {

@@ -9,6 +9,7 @@
 // the 9 row-merged spans of simkernels.js:62-74) are found once, by computeL,
 // and stored as a slot-major list; every later sweep walks that list instead of
 // re-testing the ~216 candidates of the 27-cell stencil.
+#include "xpbi_mlp.hpp"
 #include "xpbi_nbr.hpp"
 
 namespace xpbi {
@@ -54,7 +55,8 @@ __global__ __launch_bounds__(256) void k_solveA(KP P, const DevMat* __restrict__
                                                 const float4* __restrict__ vin, M3C Fm, M3C Lm,
                                                 float4* __restrict__ aux, M3P Sp, float4* __restrict__ selfDv,
                                                 const u32* __restrict__ cs, NLc nl, double* __restrict__ part,
-                                                const SolveCtl* __restrict__ ctl, u32 k) {
+                                                const SolveCtl* __restrict__ ctl, u32 k,
+                                                const float* __restrict__ hbHint) {
   if (skipIteration(ctl, k)) return;
   __shared__ MatLds T;
   loadMatLds(T, mats);
@@ -64,9 +66,11 @@ __global__ __launch_bounds__(256) void k_solveA(KP P, const DevMat* __restrict__
     const float4 pi = pos[i];
     const f3 vi = xyz(vin[i]);
     Moments mo;
-    const bool solid = (T.solidMask >> matOfTag(pi.w)) & 1u;
+    const u32 mi = matOfTag(pi.w);
+    const bool solid = (T.solidMask >> mi) & 1u;
     if (solid) mo = neighbourMoments<true>(P, cs, pos, vin, nl, T.invMass, T.solidMask, i, xyz(pi), vi);
-    solveAFinish(P, T, i, pi, vi, solid, mo, Fm, Lm, aux, Sp, selfDv, &r2, &c2);
+    const float hint = (hbHint && T.m[mi].kind == MAT_HB) ? hbHint[i] : -1.0f;
+    solveAFinish(P, T, i, pi, vi, solid, mo, Fm, Lm, aux, Sp, selfDv, &r2, &c2, hint);
   }
   if (part) blockReduce2((double)r2, (double)c2, part + 2 * blockIdx.x);
 }
@@ -90,8 +94,8 @@ __global__ __launch_bounds__(256) void k_resid_final(const double* __restrict__ 
       out[1] = s_out[1];
     }
     if (ctl) {
-      if (k == 0) ctl->c0sq = s_out[1];
-      const double c0 = (k == 0) ? s_out[1] : ctl->c0sq;
+      if (k == 0 && !ctl->c0preset) ctl->c0sq = s_out[1];
+      const double c0 = ctl->c0sq;
       ctl->iters = k + 1;
       ctl->r2 = s_out[0];
       if (c0 <= 0.0 || s_out[0] <= tol2 * c0) ctl->done = 1u;
@@ -103,6 +107,7 @@ __global__ void k_ctl_reset(SolveCtl* __restrict__ ctl) {
   if (threadIdx.x == 0) {
     ctl->done = 0u;
     ctl->iters = 0u;
+    ctl->c0preset = 0u;
     ctl->c0sq = 0.0;
     ctl->r2 = 0.0;
   }
@@ -246,7 +251,8 @@ __global__ __launch_bounds__(256) void k_xsph(KP P, const DevMat* __restrict__ m
 // simkernels.js:510-525
 __global__ __launch_bounds__(256) void k_updateF(KP P, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
                                                  const float4* __restrict__ vin, M3P Fm, M3C Lm,
-                                                 float4* __restrict__ aux, const u32* __restrict__ cs, NLc nl) {
+                                                 float4* __restrict__ aux, const u32* __restrict__ cs, NLc nl,
+                                                 const float* __restrict__ hbHint) {
   __shared__ MatLds T;
   loadMatLds(T, mats);
   const u32 i = xcdBlock() * 256u + threadIdx.x;
@@ -263,7 +269,8 @@ __global__ __launch_bounds__(256) void k_updateF(KP P, const DevMat* __restrict_
   f33 Ftr = mul(madd(mident<float>(), mscale(gradV, P.dt)), ldm(Fm, i));
   SVD3<float> sv = svd3_guarded(Ftr);
   float st;
-  f3 sig = returnMap(sv.sig, m, state, P.dt, &st);
+  const float hint = (hbHint && m.kind == MAT_HB) ? hbHint[i] : -1.0f;
+  f3 sig = returnMap(sv.sig, m, state, P.dt, &st, hint);
   stm(Fm, i, diagMul(sv.U, safeSig(sig), sv.V));
   aux[i].z = st;
 }
@@ -282,6 +289,119 @@ __global__ __launch_bounds__(256) void k_integrate(KP P, float4* __restrict__ po
   f3 x = xyz(p) + P.dt * v;
   x = f3{wclamp(x.x, P.dminx, P.dmaxx), wclamp(x.y, P.dminy, P.dmaxy), wclamp(x.z, P.dminz, P.dmaxz)};
   pos[i] = make_float4(x.x, x.y, x.z, p.w);
+}
+
+// ---------------------------------------------------------------- learned warm start
+// At lambda = 0 (before the first Jacobi iteration): the constraint of every
+// solid particle as solveA evaluates it, its warm-start features, the plain
+// first step d0 = (-C - gamma dt dC.v) / ((1 + gamma) denom + alpha~), and the
+// unscaled Sp = GL, selfDv = sgrad that the MLP epilogue scales by lambda0.
+// Feature row (16 floats): 0-2 100 log sigma of the trial F (descending),
+// 3 100 tr log sigma and 4 100 |dev log sigma| after the return map,
+// 5 log10(mu) - 5, 6 lambda / (lambda + 2 mu), 7 material kind / 6,
+// 8 strength (DP alpha; VM, HB 100 sigmaY / mu; NACC xi / 10; snow 10 theta_c),
+// 9 support neighbours / 40, 10 clamp(previous substep's lambda / d0, +-8),
+// 11 alpha~ / ((1 + gamma) denom + alpha~); 12 d0 (0: no active constraint),
+// 13 HB root position of the return map (NaN: none), 14 C, 15 0.
+// Also reduces sum C^2 (the residual's denominator) per block.
+__global__ __launch_bounds__(256) void k_features(KP P, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
+                                                  const float4* __restrict__ vin, M3C Fm, M3C Lm,
+                                                  const float4* __restrict__ aux, M3P Sp, float4* __restrict__ selfDv,
+                                                  const u32* __restrict__ cs, NLc nl,
+                                                  const float* __restrict__ lamPrev, float* __restrict__ feat,
+                                                  double* __restrict__ part) {
+  __shared__ MatLds T;
+  loadMatLds(T, mats);
+  const u32 i = xcdBlock() * 256u + threadIdx.x;
+  float c2 = 0.0f;
+  if (i < P.N) {
+    float row[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) row[k] = 0.0f;
+    row[13] = __builtin_nanf("");
+    const float4 pi = pos[i];
+    const u32 mi = matOfTag(pi.w);
+    bool wrote = false;
+    if ((T.solidMask >> mi) & 1u) {
+      const f3 vi = xyz(vin[i]);
+      const Moments mo = neighbourMoments<true>(P, cs, pos, vin, nl, T.invMass, T.solidMask, i, xyz(pi), vi);
+      const float4 a = aux[i];
+      const Constraint c = evalConstraint(P, T, i, pi, a, mo, Fm, Lm, -1.0f, true);
+      if (c.active) {
+        const MatV m = hardenedMat(matv(T, mi), a.z);
+        const float g = P.kDamp;
+        const float den = (1.0f + g) * c.denom + P.alphaT;
+        const float d0 = (-c.C - g * P.dt * c.cdot) / den;
+        const f3 lt = vlog(safeSig(c.sigTrial));
+        const f3 lp = vlog(safeSig(c.sigPost));
+        const float trp = lp.x + lp.y + lp.z;
+        float strength = 0.0f;
+        switch (m.kind) {
+          case MAT_DP: strength = m.p0; break;
+          case MAT_VM:
+          case MAT_HB: strength = 100.0f * m.p0 / m.mu; break;
+          case MAT_NACC: strength = 0.1f * m.p2; break;
+          case MAT_SNOW: strength = 10.0f * m.p0; break;
+          default: break;
+        }
+        row[0] = 100.0f * lt.x;
+        row[1] = 100.0f * lt.y;
+        row[2] = 100.0f * lt.z;
+        row[3] = 100.0f * trp;
+        row[4] = 100.0f * len(lp - splat3(trp / 3.0f));
+        row[5] = log10f(m.mu) - 5.0f;
+        row[6] = m.lam / (m.lam + 2.0f * m.mu);
+        row[7] = (float)m.kind / 6.0f;
+        row[8] = strength;
+        row[9] = (float)nl.cnt[i] / 40.0f;
+        row[10] = (d0 != 0.0f && lamPrev) ? wclamp(lamPrev[i] / d0, -8.0f, 8.0f) : 0.0f;
+        row[11] = P.alphaT / den;
+        row[12] = d0;
+        row[13] = c.hbT;
+        row[14] = c.C;
+        stm(Sp, i, c.GL);
+        selfDv[i] = make_float4(c.sgrad.x, c.sgrad.y, c.sgrad.z, 0.0f);
+        c2 = c.C * c.C;
+        wrote = true;
+      }
+    }
+    if (!wrote) {
+      stm(Sp, i, mzero<float>());
+      selfDv[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    float4* o = (float4*)(feat + (u64)i * kFeatStride);
+    o[0] = make_float4(row[0], row[1], row[2], row[3]);
+    o[1] = make_float4(row[4], row[5], row[6], row[7]);
+    o[2] = make_float4(row[8], row[9], row[10], row[11]);
+    o[3] = make_float4(row[12], row[13], row[14], row[15]);
+  }
+  blockReduce2(0.0, (double)c2, part + 2 * blockIdx.x);
+}
+
+// sum C^2 of the features pass becomes the residual's denominator
+__global__ __launch_bounds__(256) void k_resid_c0(const double* __restrict__ part, u32 nb, SolveCtl* __restrict__ ctl) {
+  __shared__ double s_out[2];
+  double a = 0.0, b = 0.0;
+  for (u32 j = threadIdx.x; j < nb; j += 256u) {
+    a += part[2 * j];
+    b += part[2 * j + 1];
+  }
+  blockReduce2(a, b, s_out);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ctl->c0sq = s_out[1];
+    ctl->c0preset = 1u;
+  }
+}
+
+// training targets after the plain solve: lambda / d0 and the HB root position
+__global__ __launch_bounds__(256) void k_collect_targets(const float* __restrict__ feat, const float4* __restrict__ aux,
+                                                         u32 N, float* __restrict__ tgt) {
+  const u32 i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= N) return;
+  const float d0 = feat[(u64)i * kFeatStride + 12];
+  tgt[2 * i] = d0 != 0.0f ? aux[i].y / d0 : __builtin_nanf("");
+  tgt[2 * i + 1] = feat[(u64)i * kFeatStride + 13];
 }
 
 // ---------------------------------------------------------------- schedule
@@ -306,6 +426,24 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
   SolveCtl* ctl = loop ? s->ctl : nullptr;
   const double tol2 = (double)s->tol * (double)s->tol;
   if (loop) hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, st, ctl);
+  const float* hbHint = s->wsMode == 1 ? (const float*)s->wsHb : nullptr;
+  if (s->wsMode != 0) {
+    hipLaunchKernelGGL(k_features, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
+                       (const float4*)s->vel[vi], cm(F), cm(s->L), (const float4*)S.aux, s->Sp, s->selfDv,
+                       (const u32*)s->cellStart, nl, (const float*)s->lamPrev, s->wsFeat, s->residPart);
+    s->prof_mark("features");
+    if (loop) hipLaunchKernelGGL(k_resid_c0, dim3(1), dim3(256), 0, st, (const double*)s->residPart, gN, ctl);
+    if (s->wsMode == 1) {
+      // lambda0 from the MLP, then its velocity change through the constraint gather
+      launch_mlp_apply(s);
+      s->prof_mark("mlp");
+      hipLaunchKernelGGL(k_solveB, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
+                         (const float4*)s->vel[vi], cm(s->Sp), (const float4*)s->selfDv, (const u32*)s->cellStart,
+                         nl, s->vel[vi ^ 1], (const SolveCtl*)nullptr, 0u);
+      s->prof_mark("solveB");
+      vi ^= 1;
+    }
+  }
   int launched = 0;
   for (int k = 0; k < iterations; k++) {
     if (loop && s->checkEvery > 0 && k > 0 && k % s->checkEvery == 0) {
@@ -318,7 +456,7 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
     double* part = (out || loop) ? s->residPart : nullptr;
     hipLaunchKernelGGL(k_solveA, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
                        (const float4*)s->vel[vi], cm(F), cm(s->L), S.aux, s->Sp, s->selfDv,
-                       (const u32*)s->cellStart, nl, part, (const SolveCtl*)ctl, (u32)k);
+                       (const u32*)s->cellStart, nl, part, (const SolveCtl*)ctl, (u32)k, hbHint);
     s->prof_mark("solveA");
     if (part) {
       hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(256), 0, st, (const double*)part, gN, out, ctl, (u32)k, tol2);
@@ -347,6 +485,9 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
     s->prof_mark("parityFix");
   }
   s->lastIters = launched;
+  if (s->wsMode == 2)
+    hipLaunchKernelGGL(k_collect_targets, dim3(gN), dim3(256), 0, st, (const float*)s->wsFeat,
+                       (const float4*)S.aux, s->N, s->wsTarget);
   if (stop == XPBI_STAGE_SOLVE) { s->vcur = vi; return; }
   hipLaunchKernelGGL(k_xsph, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
                      (const float4*)s->vel[vi], (const u32*)s->cellStart, nl, s->vel[vi ^ 1]);
@@ -354,7 +495,7 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
   vi ^= 1;
   if (stop == XPBI_STAGE_XSPH) { s->vcur = vi; return; }
   hipLaunchKernelGGL(k_updateF, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
-                     (const float4*)s->vel[vi], F, cm(s->L), S.aux, (const u32*)s->cellStart, nl);
+                     (const float4*)s->vel[vi], F, cm(s->L), S.aux, (const u32*)s->cellStart, nl, hbHint);
   s->prof_mark("updateF");
   if (stop == XPBI_STAGE_UPDATE_F) { s->vcur = vi; return; }
   hipLaunchKernelGGL(k_integrate, dim3(gN), dim3(256), 0, st, P, S.pos, s->vel[vi]);

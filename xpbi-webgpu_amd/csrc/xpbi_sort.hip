@@ -182,7 +182,8 @@ __global__ __launch_bounds__(RADIX_T) void k_radix_scatter(const u32* __restrict
 
 // ---------------------------------------------------------------- reorder
 __global__ __launch_bounds__(256) void k_reorder(KP P, const u32* __restrict__ perm, PSetC src,
-                                                 const float4* __restrict__ vsrc, PSet dst, float4* __restrict__ vdst) {
+                                                 const float4* __restrict__ vsrc, PSet dst, float4* __restrict__ vdst,
+                                                 float* __restrict__ lamPrev) {
   u32 j = xcdBlock() * 256u + threadIdx.x;
   if (j >= P.N) return;
   u32 i = perm[j];
@@ -197,6 +198,7 @@ __global__ __launch_bounds__(256) void k_reorder(KP P, const u32* __restrict__ p
   v.z = v.z + P.dt * P.grz;
   v.w = Vn;
   a.x = Vn;
+  if (lamPrev) lamPrev[j] = a.y;  // the previous substep's final lambda, for the warm start
   a.y = 0.0f;
   dst.pos[j] = p;
   vdst[j] = v;
@@ -270,7 +272,8 @@ void launch_reorder(Sim* s) {
   PSet& S = s->set[src];
   PSetC sc{S.pos, S.Fa, S.Fb, S.Fc, S.aux, S.id};
   hipLaunchKernelGGL(k_reorder, dim3(gN), dim3(256), 0, s->stream, s->kp, (const u32*)s->perm, sc,
-                     (const float4*)s->vel[s->vcur], s->set[dst], s->vel[s->vcur ^ 1]);
+                     (const float4*)s->vel[s->vcur], s->set[dst], s->vel[s->vcur ^ 1],
+                     s->wsMode != 0 ? s->lamPrev : (float*)nullptr);
   s->cur = dst;
   s->vcur ^= 1;
   s->prof_mark("reorder");

@@ -220,6 +220,24 @@ class Simulation:
         check(lib().xpbi_last_iterations(self._h, C.byref(n), C.byref(r)), self._h)
         return n.value, r.value
 
+    def set_warmstart(self, params):
+        """Learned warm start with the given network (canonical float vector of
+        MLP_PARAMS), or plain Jacobi from lambda = 0 with params = None."""
+        if params is None:
+            check(lib().xpbi_set_warmstart(self._h, None, 0), self._h)
+            return
+        p = np.ascontiguousarray(params, dtype=np.float32).ravel()
+        check(lib().xpbi_set_warmstart(self._h, _fptr(p), p.size), self._h, "set_warmstart")
+
+    def collect_warmstart_pairs(self, iterations, hasWater):
+        """One plain substep; returns (features N x 12, targets N x 2, NaN = no target)."""
+        from ._abi import MLP_INPUTS
+        x = np.zeros((self.N, MLP_INPUTS), np.float32)
+        t = np.zeros((self.N, 2), np.float32)
+        check(lib().xpbi_collect_warmstart_pairs(self._h, int(iterations), int(bool(hasWater)), _fptr(x), _fptr(t),
+                                                 self.N), self._h, "collect_warmstart_pairs")
+        return x, t
+
     def close(self):
         if getattr(self, "_h", None):
             lib().xpbi_destroy(self._h)

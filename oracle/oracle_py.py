@@ -14,7 +14,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
-SRC = [os.path.join(HERE, "xpbi_oracle.cpp"), os.path.join(HERE, "xpbi_oracle.hpp")]
+SRC = [os.path.join(HERE, "xpbi_oracle.cpp"), os.path.join(HERE, "xpbi_oracle.hpp"), os.path.abspath(__file__)]
 STAGE = dict(SORT=1, REORDER=2, COMPUTE_L=3, SOLVE=4, XSPH=5, UPDATE_F=6, FULL=7)
 
 GXX_CMD = ["g++", "-O2", "-std=c++17", "-fopenmp", "-ffp-contract=off", "-fPIC", "-shared",
@@ -45,6 +45,11 @@ def _mk_state_struct(ctype):
     return S
 
 
+class _Opts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("wsMode", C.c_int), ("wsParams", C.c_void_p), ("feat", C.c_void_p),
+                ("target", C.c_void_p), ("itersDone", C.c_int)]
+
+
 S32 = _mk_state_struct(C.c_float)
 S64 = _mk_state_struct(C.c_double)
 _lib = None
@@ -60,6 +65,14 @@ def lib():
         L.xo_substep_f32.argtypes = [C.POINTER(S32), vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp]
         L.xo_substep_f64.argtypes = [C.POINTER(S64), vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp]
         L.xo_substep_f32.restype = L.xo_substep_f64.restype = C.c_int
+        L.xo_substep_ex_f32.argtypes = [C.POINTER(S32), vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp,
+                                        C.POINTER(_Opts)]
+        L.xo_substep_ex_f64.argtypes = [C.POINTER(S64), vp, vp, C.c_int, C.c_int, C.c_int, C.c_int, vp,
+                                        C.POINTER(_Opts)]
+        L.xo_substep_ex_f32.restype = L.xo_substep_ex_f64.restype = C.c_int
+        L.xo_mlp_forward.argtypes = [vp, C.c_int, vp, vp]
+        L.xo_hb_return_map_f32.argtypes = [C.c_int, vp, C.c_float] + [vp] * 4
+        L.xo_hb_return_map_f64.argtypes = [C.c_int, vp, C.c_double] + [vp] * 4
         L.xo_kernel_f32.argtypes = [C.c_float, C.c_int] + [vp] * 6
         L.xo_kernel_f64.argtypes = [C.c_double, C.c_int] + [vp] * 6
         L.xo_svd3_f32.argtypes = [C.c_int, vp, vp, vp, vp]
@@ -121,17 +134,44 @@ class State:
                  P(self.Sp), U(self.cellTab), U(self.sortedIdx), U(self.id))
 
 
-def substep(state, params, mats, iterations, has_water, stop="FULL", nthreads=None, resid=None):
-    """Run one (partial) substep in place.  params/mats: ctypes records (bytes-compatible)."""
+def substep(state, params, mats, iterations, has_water, stop="FULL", nthreads=None, resid=None, tol=0.0,
+            warm=None, collect=False, feat=None, target=None):
+    """Run one (partial) substep in place.  params/mats: ctypes records (bytes-compatible).
+
+    tol > 0: `iterations` is a cap; the Jacobi loop stops after the first
+    iteration whose residual ||num|| / ||C at lambda = 0|| is <= tol.
+    warm: canonical MLP parameters -> learned warm start; collect: training
+    pairs.  feat (N x 16) and target (N x 2) receive the feature rows and targets
+    (allocated when None).  Returns the number of Jacobi iterations applied."""
     st = state._struct()
     nt = nthreads or os.cpu_count() or 1
     rp = resid.ctypes.data if resid is not None else None
-    f = lib().xo_substep_f32 if state.dtype == np.float32 else lib().xo_substep_f64
+    mode = 1 if warm is not None else (2 if collect else 0)
+    wp = None
+    if warm is not None:
+        wp = np.ascontiguousarray(warm, np.float32)
+        assert wp.size == 5122
+    if mode and feat is None:
+        feat = np.zeros((state.N, 16), state.dtype)
+    if mode == 2 and target is None:
+        target = np.zeros((state.N, 2), state.dtype)
+    o = _Opts(float(tol), mode, wp.ctypes.data if wp is not None else None,
+              feat.ctypes.data if feat is not None else None, target.ctypes.data if target is not None else None, 0)
+    f = lib().xo_substep_ex_f32 if state.dtype == np.float32 else lib().xo_substep_ex_f64
     rc = f(C.byref(st), C.addressof(params), C.addressof(mats), int(iterations), int(bool(has_water)),
-           STAGE[stop] if isinstance(stop, str) else int(stop), int(nt), rp)
+           STAGE[stop] if isinstance(stop, str) else int(stop), int(nt), rp, C.byref(o))
     if rc != 0:
         raise RuntimeError("oracle substep rejected the state (N/numCells mismatch)")
-    return state
+    return o.itersDone
+
+
+def mlp_forward(params, x):
+    """The C++ restatement of the warm-start MLP forward (n x 12 -> n x 2)."""
+    p = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float64).reshape(-1, 12)
+    y = np.zeros((x.shape[0], 2), np.float64)
+    lib().xo_mlp_forward(p.ctypes.data, x.shape[0], x.ctypes.data, y.ctypes.data)
+    return y
 
 
 def initial_state(scene, numCells, dtype=np.float32):
@@ -189,6 +229,17 @@ def return_map(mat, sig, state, dt, harden=False, dtype=np.float64):
     f = lib().xo_return_map_f32 if dtype == np.float32 else lib().xo_return_map_f64
     f(n, C.addressof(mat), dt, int(harden), _p(sig), _p(state), _p(so), _p(sto), _p(psi), _p(dpsi))
     return so, sto, psi, dpsi
+
+
+def hb_return_map(mat, sig, hint, dt, dtype=np.float64):
+    """HB return map with bracket hints (-1: none): (sigma out, root position or NaN)."""
+    sig = np.ascontiguousarray(sig, dtype).reshape(-1, 3)
+    n = sig.shape[0]
+    hint = np.ascontiguousarray(np.broadcast_to(np.asarray(hint, dtype), (n,)))
+    so, t = np.zeros((n, 3), dtype), np.zeros(n, dtype)
+    f = lib().xo_hb_return_map_f32 if dtype == np.float32 else lib().xo_hb_return_map_f64
+    f(n, C.addressof(mat), dt, _p(sig), _p(hint), _p(so), _p(t))
+    return so, t
 
 
 def cell_index(params, p):

@@ -53,9 +53,19 @@ struct xo_state64 {
 };
 }  // extern "C"
 
+// learned warm start / residual loop options of a substep (no reference counterpart)
+struct xo_opts {
+  double tol;             // > 0: iterations is a cap, stop at residual <= tol
+  int wsMode;             // 0 plain, 1 learned warm start, 2 collect training pairs
+  const float* wsParams;  // canonical MLP parameters (mode 1)
+  void* feat;             // N x 16 feature rows (modes 1, 2), T
+  void* target;           // N x 2 targets (mode 2), T
+  int itersDone;          // out: Jacobi iterations applied
+};
+
 template <class T, class ST>
 static int run_substep(const ST* st, const void* params, const void* mats, int iters, int hasWater, int stop,
-                       int nthreads, double* resid) {
+                       int nthreads, double* resid, xo_opts* o) {
   Sim<T> sim;
   ParamRec pr;
   std::memcpy(&pr, params, sizeof(pr));
@@ -69,17 +79,38 @@ static int run_substep(const ST* st, const void* params, const void* mats, int i
   sim.S.id = st->id;
   sim.nthreads = nthreads > 0 ? nthreads : 1;
   sim.resid = resid;
+  if (o) {
+    if ((o->wsMode == 1 && !o->wsParams) || (o->wsMode && !o->feat) || (o->wsMode == 2 && !o->target)) return -2;
+    sim.tol = o->tol;
+    sim.wsMode = o->wsMode;
+    sim.wsParams = o->wsParams;
+    sim.feat = (T*)o->feat;
+    sim.target = (T*)o->target;
+  }
   sim.substep(iters, hasWater != 0, stop);
+  if (o) o->itersDone = sim.itersDone;
   return 0;
 }
 
 extern "C" int xo_substep_f32(const xo_state32* st, const void* params, const void* mats, int iters, int hasWater, int stop,
                    int nthreads, double* resid) {
-  return run_substep<float>(st, params, mats, iters, hasWater, stop, nthreads, resid);
+  return run_substep<float>(st, params, mats, iters, hasWater, stop, nthreads, resid, nullptr);
 }
 extern "C" int xo_substep_f64(const xo_state64* st, const void* params, const void* mats, int iters, int hasWater, int stop,
                    int nthreads, double* resid) {
-  return run_substep<double>(st, params, mats, iters, hasWater, stop, nthreads, resid);
+  return run_substep<double>(st, params, mats, iters, hasWater, stop, nthreads, resid, nullptr);
+}
+extern "C" int xo_substep_ex_f32(const xo_state32* st, const void* params, const void* mats, int iters, int hasWater,
+                                 int stop, int nthreads, double* resid, xo_opts* o) {
+  return run_substep<float>(st, params, mats, iters, hasWater, stop, nthreads, resid, o);
+}
+extern "C" int xo_substep_ex_f64(const xo_state64* st, const void* params, const void* mats, int iters, int hasWater,
+                                 int stop, int nthreads, double* resid, xo_opts* o) {
+  return run_substep<double>(st, params, mats, iters, hasWater, stop, nthreads, resid, o);
+}
+// the C++ restatement of the MLP forward on n feature rows of 12 (pinned against the numpy oracle)
+extern "C" void xo_mlp_forward(const float* params, int n, const double* x, double* y) {
+  for (int i = 0; i < n; i++) mlpForward<double>(params, x + 12 * (size_t)i, y + 2 * (size_t)i);
 }
 
 // ---- unit entry points (batched over n inputs) ----
@@ -169,4 +200,22 @@ extern "C" void xo_cell_index(const void* params, int n, const float* p, u32* ou
   std::memcpy(&pr, params, sizeof(pr));
   Prm<float> P = toPrm<float>(pr);
   for (int i = 0; i < n; i++) out[i] = cellIndex(P, cellCoord(P, p[3 * i], p[3 * i + 1], p[3 * i + 2]));
+}
+
+// the HB return map with a bracket hint (learned warm start): sigOut, root position tOut
+template <class T>
+static void hb_batch(int n, const void* mat, T dt, const T* sig, const T* hint, T* sigOut, T* tOut) {
+  const Mat<T> m = toMat<T>(*(const MatRec*)mat);
+  for (int i = 0; i < n; i++) {
+    v3<T> s = rmHerschelBulkley(V(sig[3 * i], sig[3 * i + 1], sig[3 * i + 2]), m, dt, hint[i], &tOut[i]);
+    sigOut[3 * i] = s.x; sigOut[3 * i + 1] = s.y; sigOut[3 * i + 2] = s.z;
+  }
+}
+extern "C" void xo_hb_return_map_f32(int n, const void* mat, float dt, const float* sig, const float* hint,
+                                     float* sigOut, float* tOut) {
+  hb_batch<float>(n, mat, dt, sig, hint, sigOut, tOut);
+}
+extern "C" void xo_hb_return_map_f64(int n, const void* mat, double dt, const double* sig, const double* hint,
+                                     double* sigOut, double* tOut) {
+  hb_batch<double>(n, mat, dt, sig, hint, sigOut, tOut);
 }

@@ -372,8 +372,15 @@ template <class T> inline v3<T> rmVonMises(v3<T> sig, const Mat<T>& m) {
 // HB bracket override: when lo_hi != nullptr the bisection starts from
 // [lo_hi[0], lo_hi[1]] (used only by warm-start experiments).
 // common.js:244-280
-template <class T> inline v3<T> rmHerschelBulkley(v3<T> sig, const Mat<T>& m, T dt) {
+// hint in [0, 1] (learned warm start, no reference counterpart): a predicted
+// root position t in s = sY + t (sPre - sY); the bisection starts from
+// [t - 1/64, t + 1/64] when g changes sign across it (13 halvings), else from
+// the full bracket (18, common.js:264-265).  *tOut: the root position found
+// (NaN when no bisection ran).
+template <class T> inline v3<T> rmHerschelBulkley(v3<T> sig, const Mat<T>& m, T dt, T hint = T(-1),
+                                                  T* tOut = nullptr) {
   T sigY = m.p0, hexp = m.p1, eta = m.p2;
+  if (tOut) *tOut = std::numeric_limits<T>::quiet_NaN();
   v3<T> s3 = safeSig(sig);
   T J = s3.x * s3.y * s3.z;
   v3<T> bbar = s3 * s3 * std::pow(J, T(-2.0 / 3.0));
@@ -389,14 +396,25 @@ template <class T> inline v3<T> rmHerschelBulkley(v3<T> sig, const Mat<T>& m, T 
   } else {
     T ih = T(1) / hexp;
     T ei = std::pow(eta, ih);
+    auto g = [&](T x) { return ei * (x - sPre) + T(2) * muT * dt * std::pow(wmax(x - sY, T(0)), ih); };
     T lo = sY, hi = sPre;
-    for (int it = 0; it < 18; it++) {
+    int steps = 18;
+    if (hint >= T(0) && hint <= T(1)) {
+      const T a = wmax(hint - T(1.0 / 64.0), T(0)), b = wmin(hint + T(1.0 / 64.0), T(1));
+      const T lo2 = sY + a * (sPre - sY), hi2 = sY + b * (sPre - sY);
+      if ((a == T(0) || !(g(lo2) > T(0))) && (b == T(1) || g(hi2) > T(0))) {
+        lo = lo2;
+        hi = hi2;
+        steps = 13;
+      }
+    }
+    for (int it = 0; it < steps; it++) {
       T mid = T(0.5) * (lo + hi);
-      T g = ei * (mid - sPre) + T(2) * muT * dt * std::pow(wmax(mid - sY, T(0)), ih);
-      if (g > T(0)) hi = mid;
+      if (g(mid) > T(0)) hi = mid;
       else lo = mid;
     }
     sNew = T(0.5) * (lo + hi);
+    if (tOut) *tOut = (sNew - sY) / (sPre - sY);
   }
   v3<T> bNew = devb * (sNew / sPre) + splat(trb / T(3));
   bNew = vmax(bNew, T(1e-6));
@@ -485,12 +503,13 @@ template <class T> inline Mat<T> hardenedMat(Mat<T> m, T state) {
   return m;
 }
 // common.js:379-396
-template <class T> inline v3<T> returnMap(v3<T> sig, const Mat<T>& m, T state, T dt, T* stateOut) {
+template <class T> inline v3<T> returnMap(v3<T> sig, const Mat<T>& m, T state, T dt, T* stateOut,
+                                          T hbHint = T(-1), T* hbT = nullptr) {
   switch (m.kind) {
     case 1: *stateOut = state; return rmDruckerPrager(sig, m);
     case 2: return rmNACC(sig, m, state, stateOut);
     case 3: *stateOut = state; return rmVonMises(sig, m);
-    case 4: *stateOut = state; return rmHerschelBulkley(sig, m, dt);
+    case 4: *stateOut = state; return rmHerschelBulkley(sig, m, dt, hbHint, hbT);
     case 5: {
       Mat<T> mm = hardenedMat(m, state);
       v3<T> sNew = rmSnowClamp(sig, mm);
@@ -533,6 +552,43 @@ template <class T> inline v3<T> xyz(const T* a, u32 i) {
   return V(p[0], p[1], p[2]);
 }
 
+// ---- warm-start MLP forward (no reference counterpart), the model of
+// oracle/mlp_oracle.py: x = bf16(features), h1 = bf16(relu(W1 x + b1)),
+// h2 = bf16(relu(W2 h1 + b2)), y = W3 h2 + b3; weights bf16-rounded from the
+// canonical f32 vector W1[64][12], b1, W2[64][64], b2, W3[2][64], b3; sums in
+// double, rounded to float where the kernels hold f32 values.
+inline float bf16r(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return f;
+  u += 0x7fffu + ((u >> 16) & 1u);
+  u &= 0xffff0000u;
+  float r;
+  std::memcpy(&r, &u, 4);
+  return r;
+}
+template <class T> inline void mlpForward(const float* p, const T* x12, double y[2]) {
+  float x[12], h1[64], h2[64];
+  for (int f = 0; f < 12; f++) x[f] = bf16r(float(x12[f]));
+  for (int u = 0; u < 64; u++) {
+    double z = 0;
+    for (int f = 0; f < 12; f++) z += double(bf16r(p[u * 12 + f])) * double(x[f]);
+    float zf = float(z + double(p[768 + u]));
+    h1[u] = bf16r(zf > 0.0f ? zf : 0.0f);
+  }
+  for (int o = 0; o < 64; o++) {
+    double z = 0;
+    for (int i = 0; i < 64; i++) z += double(bf16r(p[832 + o * 64 + i])) * double(h1[i]);
+    float zf = float(z + double(p[4928 + o]));
+    h2[o] = bf16r(zf > 0.0f ? zf : 0.0f);
+  }
+  for (int k = 0; k < 2; k++) {
+    double z = 0;
+    for (int u = 0; u < 64; u++) z += double(bf16r(p[4992 + k * 64 + u])) * double(h2[u]);
+    y[k] = double(float(z + double(p[5120 + k])));
+  }
+}
+
 // Stage names for partial substeps (used by per-kernel parity tests)
 enum Stage { ST_SORT = 1, ST_REORDER = 2, ST_COMPUTE_L = 3, ST_SOLVE = 4, ST_XSPH = 5, ST_UPDATE_F = 6, ST_FULL = 7 };
 
@@ -544,6 +600,15 @@ template <class T> struct Sim {
   bool flip = false;  // false: velA is "in"
   // optional per-iteration residual output (sum over solids of numerator^2, C^2)
   double* resid = nullptr;  // 2 entries per iteration
+  // residual-driven loop (tol > 0: `iterations` is a cap; no reference counterpart)
+  double tol = 0;
+  int itersDone = 0;
+  // learned warm start (no reference counterpart): 0 plain, 1 learned, 2 collect pairs
+  int wsMode = 0;
+  const float* wsParams = nullptr;  // canonical MLP parameters (mode 1)
+  T* feat = nullptr;                // N x 16 feature rows (modes 1, 2)
+  T* target = nullptr;              // N x 2 targets (mode 2)
+  std::vector<T> lamPrev, hbHint;
   T* velIn() { return flip ? S.velB : S.velA; }
   T* velOut() { return flip ? S.velA : S.velB; }
 
@@ -589,6 +654,10 @@ template <class T> struct Sim {
     std::vector<u32> cursor(S.cellTab, S.cellTab + C);
     for (u32 i = 0; i < N; i++) S.sortedIdx[cursor[key[i]]++] = i;
     T* vin = velIn();
+    if (wsMode) {  // lambda at the end of the previous substep, before its reset
+      lamPrev.resize(N);
+      for (u32 i = 0; i < N; i++) lamPrev[i] = S.aux[4 * (size_t)i + 1];
+    }
 #pragma omp parallel for num_threads(nthreads) schedule(static)
     for (long ii = 0; ii < (long)N; ii++) {
       u32 i = (u32)ii;
@@ -605,8 +674,10 @@ template <class T> struct Sim {
     std::vector<T> pos(4 * (size_t)N), vel(4 * (size_t)N), aux(4 * (size_t)N), F(12 * (size_t)N);
     std::vector<u32> id(N);
     T* vin = velIn();
+    std::vector<T> lp(wsMode ? N : 0);
     for (u32 j = 0; j < N; j++) {
       u32 i = S.sortedIdx[j];
+      if (wsMode) lp[j] = lamPrev[i];
       std::memcpy(&pos[4 * (size_t)j], S.pos + 4 * (size_t)i, 4 * sizeof(T));
       std::memcpy(&vel[4 * (size_t)j], vin + 4 * (size_t)i, 4 * sizeof(T));
       std::memcpy(&aux[4 * (size_t)j], S.aux + 4 * (size_t)i, 4 * sizeof(T));
@@ -623,10 +694,12 @@ template <class T> struct Sim {
         S.F[12 * (size_t)j + 4 * c + 3] = T(0);
       }
     std::memcpy(S.id, id.data(), N * sizeof(u32));
+    if (wsMode) lamPrev.swap(lp);
   }
   // simkernels.js:189-221
   void computeL() {
     const T* vin = velIn();
+    nbrCnt.assign(P.N, 0u);
 #pragma omp parallel for num_threads(nthreads) schedule(dynamic, 256)
     for (long ii = 0; ii < (long)P.N; ii++) {
       u32 i = (u32)ii;
@@ -636,6 +709,7 @@ template <class T> struct Sim {
         if (b != i) {
           v3<T> dxv = xi - xyz(S.pos, b);
           if (dot(dxv, dxv) < P.support * P.support) {
+            nbrCnt[i]++;
             v3<T> g = kernelGradW(P, dxv);
             v3<T> r = xyz(S.pos, b) - xi;
             A = madd(A, mscale(outer(g, r), vin[4 * (size_t)b + 3]));
@@ -673,56 +747,133 @@ template <class T> struct Sim {
     T* p = S.Sp + 12 * (size_t)i;
     for (int j = 0; j < 3; j++) { p[4 * j] = m.c[j].x; p[4 * j + 1] = m.c[j].y; p[4 * j + 2] = m.c[j].z; p[4 * j + 3] = s[j]; }
   }
-  // simkernels.js:248-309  (+ residual numerator for the convergence monitor)
-  void solveA(int iter) {
-    const T* vin = velIn();
+  // HB bracket hint of particle i (learned warm start), -1 for none
+  T hintOf(u32 i) const {
+    return (wsMode == 1 && !hbHint.empty() && M[matOf(i) & 7].kind == MAT_HB) ? hbHint[i] : T(-1);
+  }
+  // simkernels.js:255-300 for solid particle i at the current velocity: trial F,
+  // SVD, return map, C and, when C >= 1e-7, GL and the second sweep's denom,
+  // sgrad and cdot
+  struct Cons {
+    bool active;
+    T C, denom, cdot, hbT;
+    m3<T> GL;
+    v3<T> sgrad, sigTrial, sigPost;
+  };
+  Cons constraintAt(u32 i, T hint, bool wantHbT) const {
+    Cons c;
+    c.active = false;
+    const T* vin = flip ? S.velB : S.velA;
+    u32 mi = matOf(i);
+    Mat<T> m = hardenedMat(M[mi & 7], S.aux[4 * (size_t)i + 2]);
+    v3<T> xi = xyz(S.pos, i);
+    m3<T> Fn = loadM(S.F, i);
+    m3<T> gradV = velocityGradient(i, xi);
+    m3<T> Ftr = mul(madd(mident<T>(), mscale(gradV, P.dt)), Fn);
+    SVD<T> sv = svd3(Ftr);
+    T st;
+    c.hbT = std::numeric_limits<T>::quiet_NaN();
+    v3<T> sig = returnMap(sv.sig, m, S.aux[4 * (size_t)i + 2], P.dt, &st, hint, wantHbT ? &c.hbT : nullptr);
+    c.sigTrial = sv.sig;
+    c.sigPost = sig;
+    T psi = stvkPsi(sig, m.mu, m.lam);
+    c.C = std::sqrt(wmax(T(2) * psi, T(0)));
+    if (c.C < T(1e-7)) return c;
+    c.active = true;
+    v3<T> dpsi = stvkDPsi(sig, m.mu, m.lam);
+    m3<T> Pk = diagMul(sv.U, dpsi / c.C, sv.V);
+    c.GL = mul(mul(Pk, transpose(Fn)), loadM(S.L, i));
+    T denom = 0, cdot = 0;
+    v3<T> sgrad = splat<T>(0);
+    forRows(xi, [&](u32 b) {
+      if (b != i && isSolid(matOf(b))) {
+        v3<T> dxv = xi - xyz(S.pos, b);
+        if (dot(dxv, dxv) < P.support * P.support) {
+          v3<T> vb = xyz(vin, b);
+          v3<T> gb = vin[4 * (size_t)b + 3] * mul(c.GL, kernelGradW(P, dxv));
+          denom = denom + dot(gb, gb) / M[matOf(b) & 7].mass;
+          sgrad = sgrad - gb;
+          cdot = cdot + dot(gb, vb);
+        }
+      }
+    });
+    c.denom = denom + dot(sgrad, sgrad) / m.mass;
+    c.cdot = cdot + dot(sgrad, xyz(vin, i));
+    c.sgrad = sgrad;
+    return c;
+  }
+  T alphaT() const { return (T(1) / P.V0) / (P.dt * P.dt); }
+  // simkernels.js:248-309  (+ residual numerator for the convergence monitor).
+  // dlamOverride (the warm start's lambda0) replaces the Jacobi step and the
+  // HB hint is not used.
+  double lastR2 = 0, lastC2 = 0;
+  void solveA(int iter, const T* dlamOverride = nullptr) {
     double r2 = 0, c2 = 0;
 #pragma omp parallel for num_threads(nthreads) schedule(dynamic, 256) reduction(+ : r2, c2)
     for (long ii = 0; ii < (long)P.N; ii++) {
       u32 i = (u32)ii;
-      u32 mi = matOf(i);
-      if (!isSolid(mi)) { storeSpSelf(i, mzero<T>(), splat<T>(0)); continue; }
-      Mat<T> m = hardenedMat(M[mi & 7], S.aux[4 * (size_t)i + 2]);
-      v3<T> xi = xyz(S.pos, i);
-      m3<T> Fn = loadM(S.F, i);
-      m3<T> gradV = velocityGradient(i, xi);
-      m3<T> Ftr = mul(madd(mident<T>(), mscale(gradV, P.dt)), Fn);
-      SVD<T> sv = svd3(Ftr);
-      T st;
-      v3<T> sig = returnMap(sv.sig, m, S.aux[4 * (size_t)i + 2], P.dt, &st);
-      T psi = stvkPsi(sig, m.mu, m.lam);
-      T C = std::sqrt(wmax(T(2) * psi, T(0)));
-      if (C < T(1e-7)) { storeSpSelf(i, mzero<T>(), splat<T>(0)); continue; }
-      v3<T> dpsi = stvkDPsi(sig, m.mu, m.lam);
-      m3<T> Pk = diagMul(sv.U, dpsi / C, sv.V);
-      m3<T> GL = mul(mul(Pk, transpose(Fn)), loadM(S.L, i));
-      T denom = 0, cdot = 0;
-      v3<T> sgrad = splat<T>(0);
-      forRows(xi, [&](u32 b) {
-        if (b != i && isSolid(matOf(b))) {
-          v3<T> dxv = xi - xyz(S.pos, b);
-          if (dot(dxv, dxv) < P.support * P.support) {
-            v3<T> vb = xyz(vin, b);
-            v3<T> gb = vin[4 * (size_t)b + 3] * mul(GL, kernelGradW(P, dxv));
-            denom = denom + dot(gb, gb) / M[matOf(b) & 7].mass;
-            sgrad = sgrad - gb;
-            cdot = cdot + dot(gb, vb);
-          }
-        }
-      });
-      denom = denom + dot(sgrad, sgrad) / m.mass;
-      cdot = cdot + dot(sgrad, xyz(vin, i));
-      T alphaT = (T(1) / P.V0) / (P.dt * P.dt);
+      if (!isSolid(matOf(i))) { storeSpSelf(i, mzero<T>(), splat<T>(0)); continue; }
+      const Cons c = constraintAt(i, dlamOverride ? T(-1) : hintOf(i), false);
+      if (!c.active) { storeSpSelf(i, mzero<T>(), splat<T>(0)); continue; }
       T g = P.kDamp;
       T lam = S.aux[4 * (size_t)i + 1];
-      T num = -C - alphaT * lam - g * P.dt * cdot;
-      T dlam = P.omega * num / ((T(1) + g) * denom + alphaT);
+      T num = -c.C - alphaT() * lam - g * P.dt * c.cdot;
+      T dlam = dlamOverride ? dlamOverride[i] : P.omega * num / ((T(1) + g) * c.denom + alphaT());
       S.aux[4 * (size_t)i + 1] = lam + dlam;
-      storeSpSelf(i, mscale(GL, dlam), sgrad * dlam);
+      storeSpSelf(i, mscale(c.GL, dlam), c.sgrad * dlam);
       r2 += double(num) * double(num);
-      c2 += double(C) * double(C);
+      c2 += double(c.C) * double(c.C);
     }
-    if (resid) { resid[2 * iter] = r2; resid[2 * iter + 1] = c2; }
+    lastR2 = r2;
+    lastC2 = c2;
+    if (resid && iter >= 0) { resid[2 * iter] = r2; resid[2 * iter + 1] = c2; }
+  }
+  // ---- learned warm start (no reference counterpart)
+  // feature rows at lambda = 0; returns sum C^2 over active solids
+  std::vector<u32> nbrCnt;  // support neighbours per particle (computeL)
+  double features() {
+    double c2 = 0;
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 256) reduction(+ : c2)
+    for (long ii = 0; ii < (long)P.N; ii++) {
+      u32 i = (u32)ii;
+      T* row = feat + 16 * (size_t)i;
+      for (int k = 0; k < 16; k++) row[k] = T(0);
+      row[13] = std::numeric_limits<T>::quiet_NaN();
+      u32 mi = matOf(i);
+      if (!isSolid(mi)) continue;
+      const Cons c = constraintAt(i, T(-1), true);
+      if (!c.active) continue;
+      Mat<T> m = hardenedMat(M[mi & 7], S.aux[4 * (size_t)i + 2]);
+      T g = P.kDamp;
+      T den = (T(1) + g) * c.denom + alphaT();
+      T d0 = (-c.C - g * P.dt * c.cdot) / den;
+      v3<T> lt = vlog(safeSig(c.sigTrial)), lp = vlog(safeSig(c.sigPost));
+      T trp = lp.x + lp.y + lp.z;
+      T strength = 0;
+      switch (m.kind) {
+        case MAT_DP: strength = m.p0; break;
+        case MAT_VM:
+        case MAT_HB: strength = T(100) * m.p0 / m.mu; break;
+        case MAT_NACC: strength = T(0.1) * m.p2; break;
+        case MAT_SNOW: strength = T(10) * m.p0; break;
+        default: break;
+      }
+      row[0] = T(100) * lt.x; row[1] = T(100) * lt.y; row[2] = T(100) * lt.z;
+      row[3] = T(100) * trp;
+      row[4] = T(100) * length(lp - splat(trp / T(3)));
+      row[5] = std::log10(m.mu) - T(5);
+      row[6] = m.lam / (m.lam + T(2) * m.mu);
+      row[7] = T(m.kind) / T(6);
+      row[8] = strength;
+      row[9] = T(nbrCnt[i]) / T(40);
+      row[10] = (d0 != T(0)) ? wclamp(lamPrev[i] / d0, T(-8), T(8)) : T(0);
+      row[11] = alphaT() / den;
+      row[12] = d0;
+      row[13] = c.hbT;
+      row[14] = c.C;
+      c2 += double(c.C) * double(c.C);
+    }
+    return c2;
   }
   // simkernels.js:369-404
   v3<T> applyBoundary(v3<T> xcIn, v3<T> vIn, v3<T> xn, T fric) const {
@@ -757,7 +908,8 @@ template <class T> struct Sim {
     return vv;
   }
   // simkernels.js:315-366
-  void solveB() {
+  // contacts = false: the warm start's impulse of lambda0 alone (no contacts or boundaries)
+  void solveB(bool contacts = true) {
     const T* vin = velIn();
     T* vout = velOut();
 #pragma omp parallel for num_threads(nthreads) schedule(dynamic, 256)
@@ -780,6 +932,7 @@ template <class T> struct Sim {
             v3<T> dxp = pb - xi;
             if (dot(dxp, dxp) < P.support * P.support) {
               dv = dv + Vi * mul(loadM(S.Sp, b), kernelGradW(P, dxp));
+              if (!contacts) return;
               v3<T> xcb = pb + P.dt * xyz(vin, b);
               v3<T> dxc = xci - xcb;
               T dist = length(dxc);
@@ -796,8 +949,11 @@ template <class T> struct Sim {
       T dvl = length(dvv);
       T dvCap = T(0.2) * P.support / P.dt;
       if (dvl > dvCap) dvv = dvv * (dvCap / dvl);
-      v3<T> v = xyz(vin, i) + dvv + corr * (P.contactOmega / P.dt);
-      v = applyBoundary(xi + P.dt * v, v, xi, m.boundFriction);
+      v3<T> v = xyz(vin, i) + dvv;
+      if (contacts) {
+        v = v + corr * (P.contactOmega / P.dt);
+        v = applyBoundary(xi + P.dt * v, v, xi, m.boundFriction);
+      }
       vout[4 * (size_t)i] = v.x; vout[4 * (size_t)i + 1] = v.y; vout[4 * (size_t)i + 2] = v.z;
       vout[4 * (size_t)i + 3] = vin[4 * (size_t)i + 3];
     }
@@ -910,7 +1066,7 @@ template <class T> struct Sim {
       m3<T> Ftr = mul(madd(mident<T>(), mscale(gradV, P.dt)), loadM(S.F, i));
       SVD<T> sv = svd3(Ftr);
       T st;
-      v3<T> sig = returnMap(sv.sig, m, S.aux[4 * (size_t)i + 2], P.dt, &st);
+      v3<T> sig = returnMap(sv.sig, m, S.aux[4 * (size_t)i + 2], P.dt, &st, hintOf(i));
       storeM(S.F, i, diagMul(sv.U, safeSig(sig), sv.V));
       S.aux[4 * (size_t)i + 2] = st;
     }
@@ -941,11 +1097,42 @@ template <class T> struct Sim {
     if (stop == ST_REORDER) return;
     computeL();
     if (stop == ST_COMPUTE_L) return;
+    bool c0preset = false;
+    double c0 = 0;
+    if (wsMode) {
+      c0 = features();
+      c0preset = true;
+      if (wsMode == 1) {
+        // lambda0 = clamp(y0, +-8) d0 and its velocity change, then plain Jacobi
+        std::vector<T> lam0(P.N);
+        hbHint.assign(P.N, T(-1));
+        for (u32 i = 0; i < P.N; i++) {
+          const T* row = feat + 16 * (size_t)i;
+          double y[2];
+          mlpForward(wsParams, row, y);
+          lam0[i] = row[12] == T(0) ? T(0) : wclamp(T(float(y[0])), T(-8), T(8)) * row[12];
+          hbHint[i] = T(float(y[1]));
+        }
+        solveA(-1, lam0.data());
+        solveB(false);
+      }
+    }
+    itersDone = 0;
     for (int k = 0; k < iterations; k++) {
       solveA(k);
+      if (k == 0 && !c0preset) c0 = lastC2;
+      const double r2 = lastR2;
       solveB();
       if (hasWater) { pbfA(); pbfB(); }
+      itersDone = k + 1;
+      if (tol > 0 && (c0 <= 0 || r2 <= tol * tol * c0)) break;
     }
+    if (wsMode == 2)
+      for (u32 i = 0; i < P.N; i++) {
+        const T d0 = feat[16 * (size_t)i + 12];
+        target[2 * (size_t)i] = d0 != T(0) ? S.aux[4 * (size_t)i + 1] / d0 : std::numeric_limits<T>::quiet_NaN();
+        target[2 * (size_t)i + 1] = feat[16 * (size_t)i + 13];
+      }
     if (stop == ST_SOLVE) { finishFlip(); return; }
     xsph();
     if (stop == ST_XSPH) { finishFlip(); return; }

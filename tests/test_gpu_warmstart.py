@@ -1,0 +1,120 @@
+"""The learned warm start on the GPU against the CPU oracle.
+
+* features and training targets (xpbi_collect_warmstart_pairs) equal the oracle's
+  collect mode on the same snapshot, to the f32 conditioning of the constraint;
+* a zero network (lambda0 = 0) reproduces plain Jacobi bit for bit;
+* a learned substep (fixed iteration count, so that both stop together) matches the
+  oracle's learned substep to the bar of the plain parity tests, on DP sand and on
+  Herschel-Bulkley goo (whose return map takes the network's bracket hint).
+"""
+import numpy as np
+import pytest
+
+import mlp_oracle as mo
+import xpbi
+from xpbi import scenes
+from xpbi.mlp import init_params
+from helpers import mats_record
+from test_gpu_parity import compare, make_sim, perturbed, snapshot
+
+pytestmark = pytest.mark.gpu
+
+
+def _net(seed, rho=1.5, t=0.5):
+    """A small random network around lambda0 = rho d0 and HB hint t."""
+    rng = np.random.default_rng(seed)
+    d = mo.split(init_params(seed).astype(np.float64))
+    d["W3"] = d["W3"] * 0.05
+    d["b3"] = np.array([rho, t])
+    d["b1"] = rng.normal(0, 0.1, 64)
+    return mo.join(d).astype(np.float32)
+
+
+def _settled(name_or_scene, frames=40):
+    scene = scenes.SCENES[name_or_scene](seed=1) if isinstance(name_or_scene, str) else name_or_scene
+    sim = make_sim(scene)
+    sim.step(frames * scene["substeps"], scene["iterations"], scene["hasWater"])
+    return scene, sim
+
+
+def _rows_err(a, b, scale):
+    return np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64)) / scale
+
+
+@pytest.mark.parametrize("name", ["Sand: column collapse", "Goo: Herschel-Bulkley", "Dam breach: water + sand"])
+def test_features_and_targets_match_oracle(oracle, name):
+    scene, sim = _settled(name)
+    I, water = scene["iterations"], scene["hasWater"]
+    st32 = snapshot(sim, oracle)
+    st64 = st32.copy(np.float64)
+    x, t = sim.collect_warmstart_pairs(I, water)
+    mats = mats_record(scene)
+    f32, f64 = np.zeros((sim.N, 16), np.float32), np.zeros((sim.N, 16))
+    t32, t64 = np.zeros((sim.N, 2), np.float32), np.zeros((sim.N, 2))
+    oracle.substep(st32, sim.params, mats, I, water, collect=True, feat=f32, target=t32)
+    oracle.substep(st64, sim.params, mats, I, water, collect=True, feat=f64, target=t64)
+    active = f32[:, 12] != 0
+    print(f"{name}: N={sim.N}, active {active.sum()}", flush=True)
+    assert active.sum() > 0.2 * sim.N
+    # the same particles carry an active constraint (a handful may sit on the C = 1e-7 cut)
+    assert ((x[:, 0] != 0) != active).sum() <= max(3, 1e-3 * sim.N)
+    both = active & (x[:, 0] != 0) & (f64[:, 12] != 0)
+    scale = np.maximum(np.abs(f64[both, :12]).max(axis=0), 1e-3)
+    eg = _rows_err(x[both], f64[both, :12], scale)
+    eo = _rows_err(f32[both, :12], f64[both, :12], scale)
+    for k in range(12):
+        q, qo = np.quantile(eg[:, k], 0.999), np.quantile(eo[:, k], 0.999)
+        print(f"  feature {k}: gpu-f64 q999 {q:.2e} max {eg[:, k].max():.2e} | f32-f64 q999 {qo:.2e}", flush=True)
+        assert q <= max(1e-4, 3 * qo), k
+    # targets: lambda / d0 after the plain solve, and the HB root position
+    tv = both & np.isfinite(t64[:, 0])
+    er = np.abs(t[tv, 0] - t64[tv, 0]) / np.maximum(np.abs(t64[tv, 0]), 1.0)
+    ero = np.abs(t32[tv, 0] - t64[tv, 0]) / np.maximum(np.abs(t64[tv, 0]), 1.0)
+    print(f"  target rho: gpu q99 {np.quantile(er, 0.99):.2e} | f32 q99 {np.quantile(ero, 0.99):.2e}", flush=True)
+    assert np.quantile(er, 0.99) <= max(1e-3, 3 * np.quantile(ero, 0.99))
+    hb = tv & np.isfinite(t64[:, 1]) & np.isfinite(t[:, 1])
+    if hb.any():
+        assert np.quantile(np.abs(t[hb, 1] - t64[hb, 1]), 0.99) <= 1e-3
+    sim.close()
+
+
+@pytest.mark.parametrize("tol", [0.0, 1e-3])
+def test_zero_network_is_plain_jacobi(tol):
+    scene = scenes.sand_column(16, 32, 16, 0.01, [0.6, 0.5, 0.6], seed=5, iterations=20)
+    a = make_sim(scene)
+    a.step(230, 20, False)
+    snap = {k: a.read(k) for k in ("POS", "VEL", "F", "AUX", "ID")}
+    b = make_sim(scene)
+    for sim in (a, b):
+        for k, v in snap.items():
+            sim.write(k, v)
+        sim.set_tolerance(tol, 4)
+    a.set_warmstart(np.zeros(5122, np.float32))
+    for _ in range(3):  # consecutive substeps: the previous-lambda feature is exercised too
+        a.substep(60, False)
+        b.substep(60, False)
+        assert a.last_iterations() == b.last_iterations()
+        for k in ("POS", "VEL", "F", "AUX", "ID"):
+            assert np.array_equal(a.read(k), b.read(k)), k
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("name", ["Sand: column collapse", "Goo: Herschel-Bulkley"])
+def test_learned_substep_matches_oracle(oracle, name):
+    scene, sim = _settled(name)
+    net = _net(3)
+    I, water = scene["iterations"], scene["hasWater"]
+    sim.set_warmstart(net)
+    sim.substep(I, water)  # the previous-lambda feature now comes from a learned substep
+    st32 = snapshot(sim, oracle)
+    # the oracle needs lambda of the last substep: aux.y is in the snapshot (AUX)
+    st64 = st32.copy(np.float64)
+    variants = [st32] + [perturbed(st32, k) for k in (1, 2)]
+    mats = mats_record(scene)
+    sim.substep(I, water)
+    for st in variants + [st64]:
+        oracle.substep(st, sim.params, mats, I, water, warm=net)
+    assert np.array_equal(sim.read("PERM"), st32.sortedIdx)
+    compare(sim, variants, st64, f"learned {name}")
+    sim.close()

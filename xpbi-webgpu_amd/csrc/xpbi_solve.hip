@@ -123,7 +123,9 @@ __global__ __launch_bounds__(256) void k_parity_fix(const SolveCtl* __restrict__
   if (i < n) dst[i] = src[i];
 }
 
-// simkernels.js:315-366
+// simkernels.js:315-366.  kContacts = false (the warm start's impulse of
+// lambda0): the constraint impulse alone, no contacts or boundaries.
+template <bool kContacts>
 __global__ __launch_bounds__(256) void k_solveB(KP P, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
                                                 const float4* __restrict__ vin, M3C Sp,
                                                 const float4* __restrict__ selfDv, const u32* __restrict__ cs,
@@ -138,8 +140,9 @@ __global__ __launch_bounds__(256) void k_solveB(KP P, const DevMat* __restrict__
   const float4 vi4 = vin[i];
   const u32 mi = matOfTag(pi.w);
   f3 dv = splat3(0.0f), corr = splat3(0.0f);
-  if ((T.solidMask >> mi) & 1u) solveBGatherGlobal(P, T, i, pi, vi4, pos, vin, Sp, selfDv, cs, nl, &dv, &corr);
-  solveBFinish(P, T, i, pi, vi4, dv, corr, vout);
+  if ((T.solidMask >> mi) & 1u)
+    solveBGatherGlobal<kContacts>(P, T, i, pi, vi4, pos, vin, Sp, selfDv, cs, nl, &dv, &corr);
+  solveBFinish<kContacts>(P, T, i, pi, vi4, dv, corr, vout);
 }
 
 // ---------------------------------------------------------------- PBF
@@ -437,9 +440,9 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
       // lambda0 from the MLP, then its velocity change through the constraint gather
       launch_mlp_apply(s);
       s->prof_mark("mlp");
-      hipLaunchKernelGGL(k_solveB, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
-                         (const float4*)s->vel[vi], cm(s->Sp), (const float4*)s->selfDv, (const u32*)s->cellStart,
-                         nl, s->vel[vi ^ 1], (const SolveCtl*)nullptr, 0u);
+      hipLaunchKernelGGL(k_solveB<false>, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats,
+                         (const float4*)S.pos, (const float4*)s->vel[vi], cm(s->Sp), (const float4*)s->selfDv,
+                         (const u32*)s->cellStart, nl, s->vel[vi ^ 1], (const SolveCtl*)nullptr, 0u);
       s->prof_mark("solveB");
       vi ^= 1;
     }
@@ -462,7 +465,7 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
       hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(256), 0, st, (const double*)part, gN, out, ctl, (u32)k, tol2);
       if (out) s->residIters = k + 1;
     }
-    hipLaunchKernelGGL(k_solveB, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
+    hipLaunchKernelGGL(k_solveB<true>, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
                        (const float4*)s->vel[vi], cm(s->Sp), (const float4*)s->selfDv, (const u32*)s->cellStart,
                        nl, s->vel[vi ^ 1], (const SolveCtl*)ctl, (u32)k);
     s->prof_mark("solveB");

@@ -208,8 +208,9 @@ int xpbi_read_residuals(xpbi_sim* sim, double* out, int cap, int* count);
  * enqueuing iterations: it polls the device flag every check_every iterations
  * (one stream synchronisation each).  tol = 0 restores the fixed count. */
 int xpbi_set_tolerance(xpbi_sim* sim, float tol, int check_every);
-/* Jacobi iterations applied by the last substep and, with tol > 0, its final
- * residual r_k (-1 otherwise).  Synchronises. */
+/* Jacobi iterations applied by the last substep (a learned first iteration
+ * counts as one) and, with tol > 0, its final residual r_k (-1 otherwise).
+ * Synchronises. */
 int xpbi_last_iterations(xpbi_sim* sim, int* iterations, double* residual);
 
 /* ---- Learned warm start (no reference counterpart: the reference starts every
@@ -219,11 +220,14 @@ int xpbi_last_iterations(xpbi_sim* sim, int* iterations, double* residual);
  * A 12 -> 64 -> 64 -> 2 MLP (BF16 weights and activations, f32 accumulation;
  * canonical parameter order W1[64][12], b1[64], W2[64][64], b2[64], W3[2][64],
  * b3[2], row = output unit) maps each solid particle's features at lambda = 0 to
- * (rho, t): the warm start sets lambda0 = clamp(rho, +-8) d0, where d0 is the plain
- * first step (-C - gamma dt dC.v) / ((1 + gamma) denom + alpha~), applies the
- * matching velocity change through the constraint gather, and then runs plain
- * Jacobi (to the tolerance of xpbi_set_tolerance, measured against the same
- * ||C|| at lambda = 0 as a plain run).  t is the HB return map's predicted root
+ * (rho, t): the warm start replaces the first Jacobi iteration's step
+ * Delta-lambda = omega d0 (d0 = (-C - gamma dt dC.v) / ((1 + gamma) denom + alpha~)
+ * at lambda = 0) by lambda0 = clamp(rho, +-8) d0 and applies it like any
+ * iteration (solveB, and PBF with water); plain Jacobi then continues, to the
+ * tolerance of xpbi_set_tolerance measured against the same ||C|| at
+ * lambda = 0 as a plain run, or for the remaining iterations of a fixed count
+ * (the learned iteration counts as one).  A network that outputs rho = omega
+ * reproduces plain Jacobi exactly.  t is the HB return map's predicted root
  * position in its bisection bracket.  Features (x, 12 per particle):
  * 100 log sigma of the trial F (3, descending), 100 tr log sigma and
  * 100 |dev log sigma| after the return map, log10(mu) - 5, lambda / (lambda + 2 mu),
@@ -237,14 +241,17 @@ int xpbi_last_iterations(xpbi_sim* sim, int* iterations, double* residual);
  * with params = NULL. */
 int xpbi_set_warmstart(xpbi_sim* sim, const float* params, size_t n);
 /* Run one plain substep (like xpbi_substep) and return, per particle in the
- * order of the substep, its features x (rows x 12) and training targets t
+ * order of the substep, its features x (rows x 12), training targets t
  * (rows x 2): lambda at the end of the solve / d0 and the HB root position of
- * the first iteration's return map (NaN where there is none).  rows = N. */
-int xpbi_collect_warmstart_pairs(xpbi_sim* sim, int iterations, int has_water, float* x, float* t, size_t rows);
+ * the first iteration's return map (NaN where there is none), and w (rows, may
+ * be NULL): the particle's weight in the residual, (-C - gamma dt dC.v)^2 at
+ * lambda = 0 (0 without an active constraint).  rows = N. */
+int xpbi_collect_warmstart_pairs(xpbi_sim* sim, int iterations, int has_water, float* x, float* t, float* w,
+                                 size_t rows);
 
 /* The MLP trainer: forward and backward on BF16 MFMA, Adam (beta 0.9, 0.999,
  * eps 1e-8) on f32 master weights.  Loss = mean over rows of
- * sum_k (y_k - t_k)^2, skipping NaN targets.  One trainer per device; calls on one
+ * w_row sum_k (y_k - t_k)^2, skipping NaN targets (w = 1 without weights).  One trainer per device; calls on one
  * trainer must not overlap. */
 typedef struct xpbi_mlp xpbi_mlp;
 int xpbi_mlp_create(int device, xpbi_mlp** out);
@@ -253,8 +260,9 @@ const char* xpbi_mlp_last_error(const xpbi_mlp* mlp);
 /* Set the f32 master weights (resets the Adam moments and step count). */
 int xpbi_mlp_set_params(xpbi_mlp* mlp, const float* params, size_t n);
 int xpbi_mlp_get_params(xpbi_mlp* mlp, float* params, size_t n);
-/* Training data: x rows x 12, t rows x 2 (copied to the device). */
-int xpbi_mlp_set_data(xpbi_mlp* mlp, const float* x, const float* t, size_t rows);
+/* Training data: x rows x 12, t rows x 2, optional sample weights w (rows, NULL:
+ * all 1); copied to the device. */
+int xpbi_mlp_set_data(xpbi_mlp* mlp, const float* x, const float* t, const float* w, size_t rows);
 /* y (rows x 2) = the network on x (rows x 12). */
 int xpbi_mlp_predict(xpbi_mlp* mlp, const float* x, size_t rows, float* y);
 /* Loss and its gradient (canonical order) over data rows [first, first + count). */

@@ -8,7 +8,8 @@ so this restates the model the library defines:
     x = bf16(features), h1 = bf16(relu(W1 x + b1)), h2 = bf16(relu(W2 h1 + b2)),
     y = W3 h2 + b3,   weights bf16-rounded from f32 masters,
 
-its loss L = (1/B) sum_n sum_k m_nk (y_nk - t_nk)^2 (m = 0 where t is NaN), the
+its loss L = (1/B) sum_n w_n sum_k m_nk (y_nk - t_nk)^2 (w: sample weights, 1
+by default; m = 0 where t is NaN), the
 gradient with every operand of a weight-gradient product rounded to bf16, and
 Adam (Kingma & Ba 2015, beta 0.9 / 0.999, eps 1e-8).  Sums run in float64.
 With rounding=False the same functions compute the exact float64 network and
@@ -72,15 +73,18 @@ def forward(p, X, rounding=True):
     return y, dict(x=x, z1=z1, h1=h1, z2=z2, h2=h2, W2=W2, W3=W3)
 
 
-def loss_grad(p, X, T, rounding=True):
-    """(loss, gradient in the canonical order) over all rows of X, T."""
+def loss_grad(p, X, T, rounding=True, W=None):
+    """(loss, gradient in the canonical order) over all rows of X, T; W: sample weights."""
     y, c = forward(p, X, rounding)
     T = np.asarray(T, np.float64)
     m = ~np.isnan(T)
     B = X.shape[0]
+    w = np.ones((B, 1)) if W is None else np.asarray(W, np.float32).astype(np.float64).reshape(B, 1)
     e = np.where(m, y - np.where(m, T, 0.0), 0.0)
-    loss = float((e * e).sum() / B)
-    dy = 2.0 / B * e
+    loss = float((w * e * e).sum() / B)
+    dy = 2.0 / B * w * e
+    if rounding:
+        dy = dy.astype(np.float32).astype(np.float64)
     dyr = _r(dy, rounding)
     dz2 = (dy @ c["W3"]) * (c["z2"] > 0)
     dz2r = _r(dz2, rounding)

@@ -369,9 +369,7 @@ template <class T> inline v3<T> rmVonMises(v3<T> sig, const Mat<T>& m) {
   v3<T> en = e - (dg / dn) * dev;
   return vexp(en);
 }
-// HB bracket override: when lo_hi != nullptr the bisection starts from
-// [lo_hi[0], lo_hi[1]] (used only by warm-start experiments).
-// common.js:244-280
+// common.js:244-280.
 // hint in [0, 1] (learned warm start, no reference counterpart): a predicted
 // root position t in s = sY + t (sPre - sY); the bisection starts from
 // [t - 1/64, t + 1/64] when g changes sign across it (13 halvings), else from
@@ -871,6 +869,7 @@ template <class T> struct Sim {
       row[12] = d0;
       row[13] = c.hbT;
       row[14] = c.C;
+      row[15] = -c.C - g * P.dt * c.cdot;
       c2 += double(c.C) * double(c.C);
     }
     return c2;
@@ -908,8 +907,7 @@ template <class T> struct Sim {
     return vv;
   }
   // simkernels.js:315-366
-  // contacts = false: the warm start's impulse of lambda0 alone (no contacts or boundaries)
-  void solveB(bool contacts = true) {
+  void solveB() {
     const T* vin = velIn();
     T* vout = velOut();
 #pragma omp parallel for num_threads(nthreads) schedule(dynamic, 256)
@@ -932,7 +930,6 @@ template <class T> struct Sim {
             v3<T> dxp = pb - xi;
             if (dot(dxp, dxp) < P.support * P.support) {
               dv = dv + Vi * mul(loadM(S.Sp, b), kernelGradW(P, dxp));
-              if (!contacts) return;
               v3<T> xcb = pb + P.dt * xyz(vin, b);
               v3<T> dxc = xci - xcb;
               T dist = length(dxc);
@@ -949,11 +946,8 @@ template <class T> struct Sim {
       T dvl = length(dvv);
       T dvCap = T(0.2) * P.support / P.dt;
       if (dvl > dvCap) dvv = dvv * (dvCap / dvl);
-      v3<T> v = xyz(vin, i) + dvv;
-      if (contacts) {
-        v = v + corr * (P.contactOmega / P.dt);
-        v = applyBoundary(xi + P.dt * v, v, xi, m.boundFriction);
-      }
+      v3<T> v = xyz(vin, i) + dvv + corr * (P.contactOmega / P.dt);
+      v = applyBoundary(xi + P.dt * v, v, xi, m.boundFriction);
       vout[4 * (size_t)i] = v.x; vout[4 * (size_t)i + 1] = v.y; vout[4 * (size_t)i + 2] = v.z;
       vout[4 * (size_t)i + 3] = vin[4 * (size_t)i + 3];
     }
@@ -1099,11 +1093,13 @@ template <class T> struct Sim {
     if (stop == ST_COMPUTE_L) return;
     bool c0preset = false;
     double c0 = 0;
+    int warm = 0;
     if (wsMode) {
       c0 = features();
       c0preset = true;
-      if (wsMode == 1) {
-        // lambda0 = clamp(y0, +-8) d0 and its velocity change, then plain Jacobi
+      if (wsMode == 1 && iterations > 0) {
+        // the first Jacobi iteration with Delta-lambda = lambda0 = clamp(y0, +-8) d0
+        // in place of omega d0, then plain Jacobi
         std::vector<T> lam0(P.N);
         hbHint.assign(P.N, T(-1));
         for (u32 i = 0; i < P.N; i++) {
@@ -1114,17 +1110,19 @@ template <class T> struct Sim {
           hbHint[i] = T(float(y[1]));
         }
         solveA(-1, lam0.data());
-        solveB(false);
+        solveB();
+        if (hasWater) { pbfA(); pbfB(); }
+        warm = 1;
       }
     }
-    itersDone = 0;
-    for (int k = 0; k < iterations; k++) {
+    itersDone = warm;
+    for (int k = 0; k < iterations - warm; k++) {
       solveA(k);
       if (k == 0 && !c0preset) c0 = lastC2;
       const double r2 = lastR2;
       solveB();
       if (hasWater) { pbfA(); pbfB(); }
-      itersDone = k + 1;
+      itersDone = warm + k + 1;
       if (tol > 0 && (c0 <= 0 || r2 <= tol * tol * c0)) break;
     }
     if (wsMode == 2)

@@ -48,14 +48,15 @@ def test_predict_matches_oracle(n):
     net.close()
 
 
-@pytest.mark.parametrize("n", [100, 4096, 20000])
-def test_loss_grad_matches_oracle(n):
+@pytest.mark.parametrize("n,weighted", [(100, False), (4096, False), (20000, False), (4096, True)])
+def test_loss_grad_matches_oracle(n, weighted):
     p = _params(3)
     X, T = _data(4, n)
+    W = np.random.default_rng(5).exponential(1.0, n).astype(np.float32) if weighted else None
     net = Mlp(0, p)
-    net.set_data(X, T)
+    net.set_data(X, T, W)
     loss, g = net.loss_grad()
-    lo, go = mo.loss_grad(p, X, T)
+    lo, go = mo.loss_grad(p, X, T, W=W)
     blocks = np.cumsum([0] + [int(np.prod(s)) for _, s in mo.SHAPES])
     print(f"n={n}: loss {loss:.6e} oracle {lo:.6e}", flush=True)
     assert abs(loss - lo) <= 1e-4 * lo
@@ -65,7 +66,8 @@ def test_loss_grad_matches_oracle(n):
         assert e <= 1e-2, name
     # a sub-range of the data
     loss2, g2 = net.loss_grad(first=n // 3, count=n // 2)
-    lo2, go2 = mo.loss_grad(p, X[n // 3: n // 3 + n // 2], T[n // 3: n // 3 + n // 2])
+    sl = slice(n // 3, n // 3 + n // 2)
+    lo2, go2 = mo.loss_grad(p, X[sl], T[sl], W=None if W is None else W[sl])
     assert abs(loss2 - lo2) <= 1e-4 * lo2
     assert np.linalg.norm(g2 - go2) <= 1e-2 * np.linalg.norm(go2)
     net.close()

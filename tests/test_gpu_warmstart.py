@@ -2,7 +2,8 @@
 
 * features and training targets (xpbi_collect_warmstart_pairs) equal the oracle's
   collect mode on the same snapshot, to the f32 conditioning of the constraint;
-* a zero network (lambda0 = 0) reproduces plain Jacobi bit for bit;
+* a network that predicts rho = omega (plain Jacobi's first step) reproduces plain
+  Jacobi bit for bit;
 * a learned substep (fixed iteration count, so that both stop together) matches the
   oracle's learned substep to the bar of the plain parity tests, on DP sand and on
   Herschel-Bulkley goo (whose return map takes the network's bracket hint).
@@ -47,7 +48,7 @@ def test_features_and_targets_match_oracle(oracle, name):
     I, water = scene["iterations"], scene["hasWater"]
     st32 = snapshot(sim, oracle)
     st64 = st32.copy(np.float64)
-    x, t = sim.collect_warmstart_pairs(I, water)
+    x, t, w = sim.collect_warmstart_pairs(I, water)
     mats = mats_record(scene)
     f32, f64 = np.zeros((sim.N, 16), np.float32), np.zeros((sim.N, 16))
     t32, t64 = np.zeros((sim.N, 2), np.float32), np.zeros((sim.N, 2))
@@ -72,27 +73,43 @@ def test_features_and_targets_match_oracle(oracle, name):
     ero = np.abs(t32[tv, 0] - t64[tv, 0]) / np.maximum(np.abs(t64[tv, 0]), 1.0)
     print(f"  target rho: gpu q99 {np.quantile(er, 0.99):.2e} | f32 q99 {np.quantile(ero, 0.99):.2e}", flush=True)
     assert np.quantile(er, 0.99) <= max(1e-3, 3 * np.quantile(ero, 0.99))
+    # residual weights (-C - gamma dt dC.v)^2
+    ew = np.abs(w[both] - f64[both, 15] ** 2) / np.maximum(f64[both, 15] ** 2, 1e-30)
+    ewo = np.abs(f32[both, 15].astype(np.float64) ** 2 - f64[both, 15] ** 2) / np.maximum(f64[both, 15] ** 2, 1e-30)
+    print(f"  weight: gpu q999 {np.quantile(ew, 0.999):.2e} | f32 q999 {np.quantile(ewo, 0.999):.2e}", flush=True)
+    assert np.quantile(ew, 0.999) <= max(1e-4, 3 * np.quantile(ewo, 0.999))
+    assert (w[~active & (x[:, 0] == 0)] == 0).all()
     hb = tv & np.isfinite(t64[:, 1]) & np.isfinite(t[:, 1])
     if hb.any():
         assert np.quantile(np.abs(t[hb, 1] - t64[hb, 1]), 0.99) <= 1e-3
     sim.close()
 
 
-@pytest.mark.parametrize("tol", [0.0, 1e-3])
-def test_zero_network_is_plain_jacobi(tol):
-    scene = scenes.sand_column(16, 32, 16, 0.01, [0.6, 0.5, 0.6], seed=5, iterations=20)
+@pytest.mark.parametrize("scene_name,tol", [(None, 0.0), (None, 1e-3), ("Dam breach: water + sand", 1e-3)])
+def test_omega_network_is_plain_jacobi(scene_name, tol):
+    """A network predicting rho = omega makes the learned first iteration plain Jacobi's first:
+    consecutive substeps agree bit for bit and count the same iterations."""
+    if scene_name is None:
+        scene = scenes.sand_column(16, 32, 16, 0.01, [0.6, 0.5, 0.6], seed=5, iterations=20)
+        settle = 230
+    else:
+        scene = scenes.SCENES[scene_name](seed=1)
+        settle = 20 * scene["substeps"]
+    water = scene["hasWater"]
     a = make_sim(scene)
-    a.step(230, 20, False)
+    a.step(settle, scene["iterations"], water)
     snap = {k: a.read(k) for k in ("POS", "VEL", "F", "AUX", "ID")}
     b = make_sim(scene)
     for sim in (a, b):
         for k, v in snap.items():
             sim.write(k, v)
         sim.set_tolerance(tol, 4)
-    a.set_warmstart(np.zeros(5122, np.float32))
+    p = np.zeros(5122, np.float32)
+    p[5120], p[5121] = 0.5, -1.0  # y = (omega, no HB hint)
+    a.set_warmstart(p)
     for _ in range(3):  # consecutive substeps: the previous-lambda feature is exercised too
-        a.substep(60, False)
-        b.substep(60, False)
+        a.substep(60, water)
+        b.substep(60, water)
         assert a.last_iterations() == b.last_iterations()
         for k in ("POS", "VEL", "F", "AUX", "ID"):
             assert np.array_equal(a.read(k), b.read(k)), k
@@ -117,4 +134,42 @@ def test_learned_substep_matches_oracle(oracle, name):
         oracle.substep(st, sim.params, mats, I, water, warm=net)
     assert np.array_equal(sim.read("PERM"), st32.sortedIdx)
     compare(sim, variants, st64, f"learned {name}")
+    sim.close()
+
+
+def test_trained_network_gains():
+    """Pairs from 30 substeps of a collapsing DP column, GPU training, then the next 10
+    substeps from one snapshot: the learned warm start reaches residual 1e-4 in fewer
+    Jacobi iterations (its learned first iteration counted) than plain Jacobi from
+    lambda = 0, and ends at the same positions."""
+    from xpbi import warmstart
+    scene = scenes.sand_column(24, 48, 24, 0.01, [0.8, 0.6, 0.8], seed=11, iterations=20)
+    X, T, W = warmstart.collect(scene, 260, 30, keep=0.5)
+    W = W / W.mean()
+    p, losses = warmstart.train(X, T, W, epochs=30)
+    print(f"pairs {len(X)}, loss {losses[0]:.3e} -> {losses[-1]:.3e}", flush=True)
+    assert losses[-1] < 0.2 * losses[0]
+    sim = make_sim(scene)
+    sim.step(300, 20, False)
+    snap = {k: sim.read(k) for k in ("POS", "VEL", "F", "AUX", "ID")}
+    out = {}
+    for mode, net in (("plain", None), ("learned", p)):
+        for k, v in snap.items():
+            sim.write(k, v)
+        sim.set_warmstart(net)
+        sim.set_tolerance(1e-4, 4)
+        its = []
+        for _ in range(10):
+            sim.substep(200, False)
+            n, r = sim.last_iterations()
+            assert r <= 1e-4 and n < 200
+            its.append(n)
+        pos, ids = sim.read("POS"), sim.read("ID")
+        out[mode] = (np.mean(its), pos[np.argsort(ids)])
+        print(f"{mode}: iterations {its}", flush=True)
+    ratio = out["plain"][0] / out["learned"][0]
+    dpos = np.abs(out["plain"][1][:, :3] - out["learned"][1][:, :3]).max()
+    print(f"iteration ratio {ratio:.3f}, max position difference {dpos:.2e} m", flush=True)
+    assert ratio >= 1.15
+    assert dpos <= 1e-4
     sim.close()

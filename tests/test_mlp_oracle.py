@@ -95,6 +95,20 @@ def test_gradient_matches_autograd():
     np.testing.assert_allclose(g, gt, rtol=1e-10, atol=1e-12)
 
 
+def test_weighted_gradient_matches_finite_differences():
+    p = _params(12)
+    X, T = _data(13, n=48)
+    W = np.random.default_rng(14).exponential(1.0, 48)
+    loss, g = mo.loss_grad(p, X, T, rounding=False, W=W)
+    for k in np.random.default_rng(15).choice(mo.NPARAMS, size=40, replace=False):
+        q = p.copy()
+        q[k] += 1e-6
+        lp, _ = mo.loss_grad(q, X, T, rounding=False, W=W)
+        q[k] -= 2e-6
+        lm, _ = mo.loss_grad(q, X, T, rounding=False, W=W)
+        assert abs((lp - lm) / 2e-6 - g[k]) <= 1e-6 * max(1.0, abs(g[k])), k
+
+
 def test_rounded_gradient_close_to_exact():
     """bf16 operands perturb the gradient by well under a percent (norm-wise)."""
     p = _params(8)

@@ -91,18 +91,26 @@ def test_residual_loop_equals_fixed_count(oracle):
         assert np.array_equal(getattr(a, f), getattr(b, f)), f
 
 
+def omega_network(omega=0.5):
+    """y = (omega, -1): lambda0 = omega d0 (plain Jacobi's first step), no HB hint."""
+    p = np.zeros(5122, np.float32)
+    p[5120], p[5121] = omega, -1.0
+    return p
+
+
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
-def test_zero_network_is_plain_jacobi(oracle, dtype):
-    """lambda0 = 0 applies no impulse: the warm-started substep is the plain one, bit for bit."""
+def test_omega_network_is_plain_jacobi(oracle, dtype):
+    """A network predicting rho = omega makes the learned first iteration plain Jacobi's: the
+    substeps agree bit for bit and count the same iterations."""
     scene, P, mats, st = _landed(oracle, dtype)
-    zero = np.zeros(5122, np.float32)
     for tol in (0.0, 1e-3):
         a, b = st.copy(), st.copy()
-        na = oracle.substep(a, P, mats, 40, False, tol=tol, warm=zero)
-        nb = oracle.substep(b, P, mats, 40, False, tol=tol)
-        assert na == nb
-        for f in ("pos", "velA", "F", "aux"):
-            assert np.array_equal(getattr(a, f), getattr(b, f)), (tol, f)
+        for _ in range(2):
+            na = oracle.substep(a, P, mats, 40, False, tol=tol, warm=omega_network())
+            nb = oracle.substep(b, P, mats, 40, False, tol=tol)
+            assert na == nb
+            for f in ("pos", "velA", "F", "aux"):
+                assert np.array_equal(getattr(a, f), getattr(b, f)), (tol, f)
 
 
 def test_collect_pairs(oracle):

@@ -617,10 +617,10 @@ int xpbi_last_iterations(xpbi_sim* h, int* iterations, double* residual) {
   if (s->tol > 0.0f) {
     SolveCtl c;
     if ((r = d2h(s, &c, s->ctl, sizeof(c)))) return r;
-    *iterations = (int)c.iters;
+    *iterations = (int)c.iters + s->warmIters;
     if (residual) *residual = c.c0sq > 0.0 ? sqrt(c.r2 / c.c0sq) : 0.0;
   } else {
-    *iterations = s->lastIters;
+    *iterations = s->lastIters + s->warmIters;
     if (residual) *residual = -1.0;
   }
   return XPBI_OK;
@@ -664,7 +664,8 @@ int xpbi_set_warmstart(xpbi_sim* h, const float* params, size_t n) {
   return XPBI_OK;
 }
 
-int xpbi_collect_warmstart_pairs(xpbi_sim* h, int iterations, int has_water, float* x, float* t, size_t rows) {
+int xpbi_collect_warmstart_pairs(xpbi_sim* h, int iterations, int has_water, float* x, float* t, float* w,
+                                 size_t rows) {
   if (!h) return XPBI_ERR_ARG;
   Sim* s = &h->s;
   if (rows != s->N) return fail(s, XPBI_ERR_ARG, "rows must equal the particle count");
@@ -681,7 +682,10 @@ int xpbi_collect_warmstart_pairs(xpbi_sim* h, int iterations, int has_water, flo
   std::vector<float> f(16ull * rows);
   if ((r = d2h(s, f.data(), s->wsFeat, f.size() * sizeof(float)))) return r;
   if ((r = d2h(s, t, s->wsTarget, 2ull * rows * sizeof(float)))) return r;
-  for (size_t i = 0; i < rows; i++) memcpy(x + i * XPBI_MLP_INPUTS, &f[16 * i], XPBI_MLP_INPUTS * sizeof(float));
+  for (size_t i = 0; i < rows; i++) {
+    memcpy(x + i * XPBI_MLP_INPUTS, &f[16 * i], XPBI_MLP_INPUTS * sizeof(float));
+    if (w) w[i] = f[16 * i + 15] * f[16 * i + 15];
+  }
   return XPBI_OK;
 }
 

@@ -99,6 +99,7 @@ struct Sim {
   SolveCtl* ctl = nullptr;  // device
   SolveCtl* ctlHost = nullptr;  // pinned mirror for polling
   int lastIters = 0;        // fixed-iteration count of the last substep (tol == 0)
+  int warmIters = 0;        // 1 when the last substep's first iteration was the learned one
 
   // learned warm start (mode 1) and training-pair collection (mode 2)
   int wsMode = 0;

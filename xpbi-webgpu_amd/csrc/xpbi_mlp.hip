@@ -3,10 +3,10 @@
 // trainer's C ABI.  The model and the register layouts are defined in
 // xpbi_mlp.hpp.
 //
-// Backward of the loss L = (1/B) sum_n sum_k m_nk (y_nk - t_nk)^2 (m = 0 where
-// the target is NaN), with every operand of a weight-gradient product rounded
-// to bf16 and f32 accumulation:
-//   dy = (2/B) m (y - t);  dW3 = sum bf16(dy) h2^T;  db3 = sum bf16(dy)
+// Backward of the loss L = (1/B) sum_n w_n sum_k m_nk (y_nk - t_nk)^2 (w: the
+// sample weights, 1 without; m = 0 where the target is NaN), with every operand
+// of a weight-gradient product rounded to bf16 and f32 accumulation:
+//   dy = (2/B) w m (y - t);  dW3 = sum bf16(dy) h2^T;  db3 = sum bf16(dy)
 //   dz2 = (W3^T dy) [z2 > 0];  dW2 = sum bf16(dz2) h1^T;  db2 = sum bf16(dz2)
 //   dz1 = (W2^T bf16(dz2)) [z1 > 0];  dW1 = sum bf16(dz1) x^T;  db1 = sum bf16(dz1)
 // A block of 4 waves takes 32 samples per wave per chunk; the per-sample
@@ -76,8 +76,9 @@ struct alignas(16) WaveT {
 };
 
 __global__ __launch_bounds__(kMlpBlock) void k_mlp_train(const MlpPacked* __restrict__ Wg, const float* __restrict__ X,
-                                                         const float* __restrict__ T, const u32* __restrict__ idx,
-                                                         u64 first, u32 count, u32 chunks, float* __restrict__ part) {
+                                                         const float* __restrict__ T, const float* __restrict__ Wt,
+                                                         const u32* __restrict__ idx, u64 first, u32 count, u32 chunks,
+                                                         float* __restrict__ part) {
   __shared__ MlpPacked W;
   __shared__ WaveT S[kMlpWaves];
   __shared__ float acc[kGradLen];
@@ -99,13 +100,14 @@ __global__ __launch_bounds__(kMlpBlock) void k_mlp_train(const MlpPacked* __rest
     mlp_forward(W, x8, a);
     // output error
     float dy[2];
+    const float wn = (valid && Wt) ? Wt[row] : 1.0f;
 #pragma unroll
     for (int k = 0; k < 2; k++) {
       const float t = valid ? T[2 * row + k] : NAN;
       const bool m = valid && !isnan(t);
       const float e = m ? a.y[k] - t : 0.0f;
-      if (h == 0) loss += e * e;
-      dy[k] = scale * e;
+      if (h == 0) loss += wn * e * e;
+      dy[k] = scale * wn * e;
     }
     // dz2 = (W3^T dy) [z2 > 0], D layout
     bf16x8 dz2b[2][2];
@@ -317,7 +319,7 @@ struct xpbi_mlp {
   hipStream_t stream = nullptr;
   float *w = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr, *part = nullptr, *lossLog = nullptr;
   MlpPacked* P = nullptr;
-  float *X = nullptr, *T = nullptr;
+  float *X = nullptr, *T = nullptr, *Wt = nullptr;
   u32* idx = nullptr;
   size_t rows = 0, partBlocks = 0, lossCap = 0;
   long step = 0;
@@ -370,7 +372,8 @@ int grad_pass(xpbi_mlp* m, const u32* idx, size_t first, u32 count, u32 lossSlot
   int r = ensure_part(m, blocks);
   if (r) return r;
   hipLaunchKernelGGL(k_mlp_train, dim3(blocks), dim3(kMlpBlock), 0, m->stream, (const MlpPacked*)m->P,
-                     (const float*)m->X, (const float*)m->T, idx, (u64)first, count, ch, m->part);
+                     (const float*)m->X, (const float*)m->T, (const float*)m->Wt, idx, (u64)first, count, ch,
+                     m->part);
   hipLaunchKernelGGL(k_mlp_reduce, dim3(cdivu(kGradLen, 256)), dim3(256), 0, m->stream, (const float*)m->part,
                      blocks, m->grad, m->lossLog, lossSlot);
   return XPBI_OK;
@@ -433,7 +436,7 @@ void xpbi_mlp_destroy(xpbi_mlp* m) {
   if (!m) return;
   hipSetDevice(m->dev);
   if (m->stream) hipStreamSynchronize(m->stream);
-  void* ps[] = {m->w, m->m, m->v, m->grad, m->part, m->lossLog, m->P, m->X, m->T, m->idx};
+  void* ps[] = {m->w, m->m, m->v, m->grad, m->part, m->lossLog, m->P, m->X, m->T, m->Wt, m->idx};
   for (void* p : ps) mfree(p);
   if (m->stream) hipStreamDestroy(m->stream);
   delete m;
@@ -466,7 +469,7 @@ int xpbi_mlp_get_params(xpbi_mlp* m, float* params, size_t n) {
   return e == hipSuccess ? XPBI_OK : mhip(m, e, "get_params");
 }
 
-int xpbi_mlp_set_data(xpbi_mlp* m, const float* x, const float* t, size_t rows) {
+int xpbi_mlp_set_data(xpbi_mlp* m, const float* x, const float* t, const float* w, size_t rows) {
   if (!m) return XPBI_ERR_ARG;
   if (rows > 0 && (!x || !t)) return mfail(m, XPBI_ERR_ARG, "x and t must not be null");
   if (rows >= (1ull << 31)) return mfail(m, XPBI_ERR_ARG, "too many rows");
@@ -476,6 +479,12 @@ int xpbi_mlp_set_data(xpbi_mlp* m, const float* x, const float* t, size_t rows) 
   if ((r = upload_rows(m, &m->X, x, rows))) return r;
   hipError_t e = realloc_dev(m->T, rows * 2);
   if (e == hipSuccess) e = hipMemcpy(m->T, t, rows * 2 * sizeof(float), hipMemcpyHostToDevice);
+  if (m->Wt) hipFree(m->Wt);
+  m->Wt = nullptr;
+  if (e == hipSuccess && w) {
+    e = realloc_dev(m->Wt, rows);
+    if (e == hipSuccess) e = hipMemcpy(m->Wt, w, rows * sizeof(float), hipMemcpyHostToDevice);
+  }
   if (e == hipSuccess) e = realloc_dev(m->idx, rows);
   if (e != hipSuccess) return mhip(m, e, "set_data");
   m->rows = rows;

@@ -340,13 +340,11 @@ struct SpGather {
 
 // simkernels.js:338-355 for one neighbour: constraint gather dv += V_i S_b gradW(x_b - x_i)
 // and the 0.75h contact on predicted positions
-template <bool kContacts>
 __device__ __forceinline__ void solveBPair(const KP& P, const MatLds& T, float Vi, float mass_i, f3 xci, f3 dxv,
                                            float r2, float4 pb4, float4 vb, const f33& Spb, f3* dv, f3* corr) {
   if (solidTag(T.solidMask, pb4.w)) {
     const f3 dxp = -dxv;  // x_b - x_i
     *dv = *dv + Vi * mul(Spb, gradWfast(P, dxp, r2));
-    if (!kContacts) return;
     const f3 xcb = xyz(pb4) + P.dt * xyz(vb);
     const f3 dxc = xci - xcb;
     const float d2 = dot(dxc, dxc);
@@ -361,7 +359,6 @@ __device__ __forceinline__ void solveBPair(const KP& P, const MatLds& T, float V
   }
 }
 
-template <bool kContacts>
 __device__ __forceinline__ void solveBGatherGlobal(const KP& P, const MatLds& T, u32 i, float4 pi, float4 vi4,
                                                    const float4* __restrict__ pos, const float4* __restrict__ vin,
                                                    M3C Sp, const float4* __restrict__ selfDv,
@@ -374,13 +371,11 @@ __device__ __forceinline__ void solveBGatherGlobal(const KP& P, const MatLds& T,
   forSupportG(P, cs, nl, i, xi, [&](u32 b) { return SpGather{pos[b], vin[b], Sp.a[b], Sp.b[b], Sp.c[b]}; },
               [&](u32 b, const SpGather& g, f3 dxv, float r2) {
                 const f33 Spb{{f3{g.a.x, g.a.y, g.a.z}, f3{g.a.w, g.b.x, g.b.y}, f3{g.b.z, g.b.w, g.c}}};
-                solveBPair<kContacts>(P, T, Vi, mass_i, xci, dxv, r2, g.p, g.v, Spb, dv, corr);
+                solveBPair(P, T, Vi, mass_i, xci, dxv, r2, g.p, g.v, Spb, dv, corr);
               });
 }
 
-// simkernels.js:357-365: impulse cap, contact relaxation, boundaries (the
-// impulse alone without kContacts)
-template <bool kContacts>
+// simkernels.js:357-365: impulse cap, contact relaxation, boundaries
 __device__ __forceinline__ void solveBFinish(const KP& P, const MatLds& T, u32 i, float4 pi, float4 vi4, f3 dv,
                                              f3 corr, float4* __restrict__ vout) {
   const u32 mi = matOfTag(pi.w);
@@ -389,11 +384,8 @@ __device__ __forceinline__ void solveBFinish(const KP& P, const MatLds& T, u32 i
   f3 dvv = dv / (mass_i * P.dt);
   float dvl = len(dvv);
   if (dvl > P.dvCap) dvv = dvv * (P.dvCap / dvl);
-  f3 v = xyz(vi4) + dvv;
-  if (kContacts) {
-    v = v + corr * (P.contactOmega / P.dt);
-    v = applyBoundary(P, xi + P.dt * v, v, xi, T.m[mi].boundFriction);
-  }
+  f3 v = xyz(vi4) + dvv + corr * (P.contactOmega / P.dt);
+  v = applyBoundary(P, xi + P.dt * v, v, xi, T.m[mi].boundFriction);
   vout[i] = make_float4(v.x, v.y, v.z, vi4.w);
 }
 

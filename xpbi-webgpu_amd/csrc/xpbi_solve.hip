@@ -123,9 +123,7 @@ __global__ __launch_bounds__(256) void k_parity_fix(const SolveCtl* __restrict__
   if (i < n) dst[i] = src[i];
 }
 
-// simkernels.js:315-366.  kContacts = false (the warm start's impulse of
-// lambda0): the constraint impulse alone, no contacts or boundaries.
-template <bool kContacts>
+// simkernels.js:315-366
 __global__ __launch_bounds__(256) void k_solveB(KP P, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
                                                 const float4* __restrict__ vin, M3C Sp,
                                                 const float4* __restrict__ selfDv, const u32* __restrict__ cs,
@@ -141,8 +139,8 @@ __global__ __launch_bounds__(256) void k_solveB(KP P, const DevMat* __restrict__
   const u32 mi = matOfTag(pi.w);
   f3 dv = splat3(0.0f), corr = splat3(0.0f);
   if ((T.solidMask >> mi) & 1u)
-    solveBGatherGlobal<kContacts>(P, T, i, pi, vi4, pos, vin, Sp, selfDv, cs, nl, &dv, &corr);
-  solveBFinish<kContacts>(P, T, i, pi, vi4, dv, corr, vout);
+    solveBGatherGlobal(P, T, i, pi, vi4, pos, vin, Sp, selfDv, cs, nl, &dv, &corr);
+  solveBFinish(P, T, i, pi, vi4, dv, corr, vout);
 }
 
 // ---------------------------------------------------------------- PBF
@@ -305,7 +303,8 @@ __global__ __launch_bounds__(256) void k_integrate(KP P, float4* __restrict__ po
 // 8 strength (DP alpha; VM, HB 100 sigmaY / mu; NACC xi / 10; snow 10 theta_c),
 // 9 support neighbours / 40, 10 clamp(previous substep's lambda / d0, +-8),
 // 11 alpha~ / ((1 + gamma) denom + alpha~); 12 d0 (0: no active constraint),
-// 13 HB root position of the return map (NaN: none), 14 C, 15 0.
+// 13 HB root position of the return map (NaN: none), 14 C, 15 the first step's
+// numerator -C - gamma dt dC.v (its square weighs the particle in the residual).
 // Also reduces sum C^2 (the residual's denominator) per block.
 __global__ __launch_bounds__(256) void k_features(KP P, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
                                                   const float4* __restrict__ vin, M3C Fm, M3C Lm,
@@ -362,6 +361,7 @@ __global__ __launch_bounds__(256) void k_features(KP P, const DevMat* __restrict
         row[12] = d0;
         row[13] = c.hbT;
         row[14] = c.C;
+        row[15] = -c.C - g * P.dt * c.cdot;
         stm(Sp, i, c.GL);
         selfDv[i] = make_float4(c.sgrad.x, c.sgrad.y, c.sgrad.z, 0.0f);
         c2 = c.C * c.C;
@@ -430,25 +430,38 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
   const double tol2 = (double)s->tol * (double)s->tol;
   if (loop) hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, st, ctl);
   const float* hbHint = s->wsMode == 1 ? (const float*)s->wsHb : nullptr;
+  int warm = 0;  // the learned first iteration ran
   if (s->wsMode != 0) {
     hipLaunchKernelGGL(k_features, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
                        (const float4*)s->vel[vi], cm(F), cm(s->L), (const float4*)S.aux, s->Sp, s->selfDv,
                        (const u32*)s->cellStart, nl, (const float*)s->lamPrev, s->wsFeat, s->residPart);
     s->prof_mark("features");
     if (loop) hipLaunchKernelGGL(k_resid_c0, dim3(1), dim3(256), 0, st, (const double*)s->residPart, gN, ctl);
-    if (s->wsMode == 1) {
-      // lambda0 from the MLP, then its velocity change through the constraint gather
+    if (s->wsMode == 1 && iterations > 0) {
+      // the first Jacobi iteration with Delta-lambda = lambda0 from the MLP
+      // instead of omega d0: its Sp, selfDv and lambda, then solveB (and PBF)
       launch_mlp_apply(s);
       s->prof_mark("mlp");
-      hipLaunchKernelGGL(k_solveB<false>, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats,
-                         (const float4*)S.pos, (const float4*)s->vel[vi], cm(s->Sp), (const float4*)s->selfDv,
-                         (const u32*)s->cellStart, nl, s->vel[vi ^ 1], (const SolveCtl*)nullptr, 0u);
+      hipLaunchKernelGGL(k_solveB, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
+                         (const float4*)s->vel[vi], cm(s->Sp), (const float4*)s->selfDv, (const u32*)s->cellStart,
+                         nl, s->vel[vi ^ 1], (const SolveCtl*)nullptr, 0u);
       s->prof_mark("solveB");
       vi ^= 1;
+      if (water) {
+        hipLaunchKernelGGL(k_pbfA, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
+                           (const float4*)s->vel[vi], S.aux, (const u32*)s->cellStart, (const SolveCtl*)nullptr, 0u);
+        s->prof_mark("pbfA");
+        hipLaunchKernelGGL(k_pbfB, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
+                           (const float4*)s->vel[vi], (const float4*)S.aux, (const u32*)s->cellStart,
+                           s->vel[vi ^ 1], (const SolveCtl*)nullptr, 0u);
+        s->prof_mark("pbfB");
+        vi ^= 1;
+      }
+      warm = 1;
     }
   }
   int launched = 0;
-  for (int k = 0; k < iterations; k++) {
+  for (int k = 0; k < iterations - warm; k++) {
     if (loop && s->checkEvery > 0 && k > 0 && k % s->checkEvery == 0) {
       // stop enqueuing once the device has flagged convergence
       hipMemcpyAsync(s->ctlHost, ctl, sizeof(SolveCtl), hipMemcpyDeviceToHost, st);
@@ -465,7 +478,7 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
       hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(256), 0, st, (const double*)part, gN, out, ctl, (u32)k, tol2);
       if (out) s->residIters = k + 1;
     }
-    hipLaunchKernelGGL(k_solveB<true>, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
+    hipLaunchKernelGGL(k_solveB, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
                        (const float4*)s->vel[vi], cm(s->Sp), (const float4*)s->selfDv, (const u32*)s->cellStart,
                        nl, s->vel[vi ^ 1], (const SolveCtl*)ctl, (u32)k);
     s->prof_mark("solveB");
@@ -488,6 +501,7 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
     s->prof_mark("parityFix");
   }
   s->lastIters = launched;
+  s->warmIters = warm;
   if (s->wsMode == 2)
     hipLaunchKernelGGL(k_collect_targets, dim3(gN), dim3(256), 0, st, (const float*)s->wsFeat,
                        (const float4*)S.aux, s->N, s->wsTarget);

@@ -83,13 +83,14 @@ SIGNATURES = {
     "xpbi_last_iterations": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "xpbi_set_warmstart": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t]),
     "xpbi_collect_warmstart_pairs": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float),
-                                               C.c_size_t]),
+                                               C.POINTER(C.c_float), C.c_size_t]),
     "xpbi_mlp_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
     "xpbi_mlp_destroy": (None, [_P]),
     "xpbi_mlp_last_error": (C.c_char_p, [_P]),
     "xpbi_mlp_set_params": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t]),
     "xpbi_mlp_get_params": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t]),
-    "xpbi_mlp_set_data": (C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_size_t]),
+    "xpbi_mlp_set_data": (C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                    C.c_size_t]),
     "xpbi_mlp_predict": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t, C.POINTER(C.c_float)]),
     "xpbi_mlp_loss_grad": (C.c_int, [_P, C.c_size_t, C.c_size_t, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "xpbi_mlp_adam_step": (C.c_int, [_P, C.POINTER(C.c_float), C.c_float]),

@@ -67,11 +67,17 @@ class Mlp:
         self._check(lib().xpbi_mlp_get_params(self._h, _f(p), p.size), "get_params")
         return p
 
-    def set_data(self, x, t):
+    def set_data(self, x, t, w=None):
+        """Training rows: features x (n x 12), targets t (n x 2, NaN = none), weights w (n, optional)."""
         x = np.ascontiguousarray(x, dtype=np.float32)
         t = np.ascontiguousarray(t, dtype=np.float32)
         assert x.ndim == 2 and x.shape[1] == MLP_INPUTS and t.shape == (x.shape[0], MLP_OUTPUTS)
-        self._check(lib().xpbi_mlp_set_data(self._h, _f(x), _f(t), x.shape[0]), "set_data")
+        wp = None
+        if w is not None:
+            w = np.ascontiguousarray(w, dtype=np.float32)
+            assert w.shape == (x.shape[0],)
+            wp = _f(w)
+        self._check(lib().xpbi_mlp_set_data(self._h, _f(x), _f(t), wp, x.shape[0]), "set_data")
         self.rows = x.shape[0]
 
     def predict(self, x):

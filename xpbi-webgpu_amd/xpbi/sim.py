@@ -230,13 +230,15 @@ class Simulation:
         check(lib().xpbi_set_warmstart(self._h, _fptr(p), p.size), self._h, "set_warmstart")
 
     def collect_warmstart_pairs(self, iterations, hasWater):
-        """One plain substep; returns (features N x 12, targets N x 2, NaN = no target)."""
+        """One plain substep; returns (features N x 12, targets N x 2 with NaN = no target,
+        residual weights N)."""
         from ._abi import MLP_INPUTS
         x = np.zeros((self.N, MLP_INPUTS), np.float32)
         t = np.zeros((self.N, 2), np.float32)
+        w = np.zeros(self.N, np.float32)
         check(lib().xpbi_collect_warmstart_pairs(self._h, int(iterations), int(bool(hasWater)), _fptr(x), _fptr(t),
-                                                 self.N), self._h, "collect_warmstart_pairs")
-        return x, t
+                                                 _fptr(w), self.N), self._h, "collect_warmstart_pairs")
+        return x, t, w
 
     def close(self):
         if getattr(self, "_h", None):

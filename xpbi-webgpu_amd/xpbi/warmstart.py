@@ -1,0 +1,117 @@
+"""Learned warm start: training pairs from the solver, GPU training, weights files.
+
+A training pair is one solid particle at the start of a substep's solve: its 12
+features (include/xpbi.h) and rho = lambda / d0, the multiplier plain Jacobi
+converges to over the first plain step.  Each pair is weighted by the
+particle's share of the residual at lambda = 0, (-C - gamma dt dC.v)^2: after a
+learned first iteration the particle's residual is about that numerator times
+(rho - rho_hat), so the weighted loss is the residual the warm start leaves.
+
+Training runs on the GPU in the library's own MLP kernels (xpbi.mlp).
+"""
+import json
+import os
+import time
+
+import numpy as np
+
+from . import scenes
+from .mlp import Mlp, init_params
+from .sim import Simulation, params_from_scene
+
+WEIGHTS_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "weights")
+
+
+def collect(scene, settle, substeps, tol=1e-4, cap=200, keep=0.25, seed=0, device=0):
+    """Run `settle` plain substeps, then `substeps` more (plain Jacobi to `tol`),
+    keeping a random fraction `keep` of each substep's active pairs.
+    Returns features (n x 12), targets (n x 2) and weights (n)."""
+    sim = Simulation(scene, device=device)
+    sim.writeParams(params_from_scene(scene))
+    I, water = scene["iterations"], scene["hasWater"]
+    if settle:
+        sim.step(settle, I, water)
+    sim.set_tolerance(tol, 4)
+    rng = np.random.default_rng(seed)
+    xs, ts, ws = [], [], []
+    for _ in range(substeps):
+        x, t, w = sim.collect_warmstart_pairs(cap, water)
+        # the warm start clamps rho to +-8: targets beyond it cannot be reached
+        ok = np.nonzero(np.isfinite(t[:, 0]) & (np.abs(t[:, 0]) <= 8.0) & (w > 0))[0]
+        if keep < 1.0:
+            ok = rng.choice(ok, size=max(1, int(len(ok) * keep)), replace=False) if len(ok) else ok
+        xs.append(x[ok])
+        ts.append(t[ok])
+        ws.append(w[ok])
+    sim.close()
+    return np.concatenate(xs), np.concatenate(ts), np.concatenate(ws)
+
+
+def training_columns():
+    """The training scenes: DP sand columns (phi 30) collapsing on the floor at the
+    two resolutions of the 128k and 1M configurations, with their own seeds --
+    (name, scene, settle substeps, collected substeps, kept fraction)."""
+    return [("DP sand column 40x80x40 r=0.01 (128,000)", scenes.config_c2(), 300, 40, 0.25),
+            ("DP sand column 40x80x40 r=0.005 (128,000)",
+             scenes.sand_column(40, 80, 40, 0.005, [0.75, 0.5, 0.75], seed=7), 420, 40, 0.25)]
+
+
+def training_set(device=0, seed=0, log=None, plan=None):
+    """Pairs of every training scene; each scene's weights are normalised to mean 1."""
+    xs, ts, ws = [], [], []
+    for k, (name, sc, settle, n, keep) in enumerate(plan or training_columns()):
+        t0 = time.time()
+        x, t, w = collect(sc, settle, n, keep=keep, seed=seed + k, device=device)
+        if log:
+            log(f"pairs: {name}: {len(x)} in {time.time() - t0:.1f} s")
+        if len(w):
+            w = w / w.mean()
+        xs.append(x)
+        ts.append(t)
+        ws.append(w)
+    return np.concatenate(xs), np.concatenate(ts), np.concatenate(ws).astype(np.float32)
+
+
+def train(X, T, W, epochs=30, batch=4096, lr=1e-3, seed=0, device=0, params=None):
+    """Minibatch Adam on the GPU; returns (parameters, per-epoch loss)."""
+    net = Mlp(device, init_params(seed) if params is None else params)
+    net.set_data(X, T, W)
+    losses = net.train(epochs=epochs, batch=batch, lr=lr, seed=seed + 1)
+    p = net.params()
+    net.close()
+    return p, losses
+
+
+def save(path, params, meta=None):
+    """Parameters as a float32 .npy, metadata beside it as JSON."""
+    np.save(path, np.asarray(params, np.float32))
+    if meta is not None:
+        with open(os.path.splitext(path)[0] + ".json", "w") as f:
+            json.dump(meta, f, indent=1)
+
+
+def load(path):
+    p = np.load(path, allow_pickle=False)
+    if p.dtype != np.float32 or p.shape != (5122,):
+        raise ValueError(f"{path}: expected 5122 float32 parameters")
+    return p
+
+
+def main():
+    import argparse
+    ap = argparse.ArgumentParser(description="Train the warm-start network on the GPU and save it.")
+    ap.add_argument("--out", default=os.path.join(WEIGHTS_DIR, "warmstart.npy"))
+    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--seed", type=int, default=0)
+    a = ap.parse_args()
+    t0 = time.time()
+    X, T, W = training_set(seed=a.seed, log=print)
+    p, losses = train(X, T, W, epochs=a.epochs, seed=a.seed)
+    os.makedirs(os.path.dirname(a.out), exist_ok=True)
+    save(a.out, p, dict(pairs=int(len(X)), epochs=a.epochs, seed=a.seed, final_loss=float(losses[-1]),
+                        seconds=round(time.time() - t0, 1)))
+    print(f"saved {a.out}: {len(X)} pairs, loss {losses[0]:.3e} -> {losses[-1]:.3e}")
+
+
+if __name__ == "__main__":
+    main()

@@ -9,12 +9,15 @@ column would skip the constraint sweeps.  A step is one substep of all
 particles; `value` = particles x substeps / wall seconds of the timed region,
 with every array resident in HBM.
 
-Also reported: the per-kernel HIP-event times of the same timed region, the
+The timed region replays the substep from a HIP graph (two captured substeps)
+under the launch shape the autotuner picks.  Also reported: per-kernel HIP-event
+times of an eager run of the same substep, the
 HBM roofline of the dominant kernel (solveA), Jacobi to residual 1e-4 -- plain
 from lambda = 0 against the learned warm start, whose network is trained in the
 run on the GPU (pairs from DP columns of 128,000 particles at r = 0.01 and
-0.005, never the 1M column itself) -- and the CPU oracle's substep on the host
-cores (a reported baseline).
+0.005, never the 1M column itself) -- the 64-scene batched ensemble (one launch
+per kernel for all scenes, graph-replayed, autotuned), and the CPU oracle's
+substep on the host cores (a reported baseline).
 """
 import argparse
 import json
@@ -107,6 +110,37 @@ def cpu_baseline(sim, scene, iterations, threads):
                        f"f32 C++ oracle, OpenMP {threads} threads, {dt:.1f} s")
 
 
+def ensemble(I, settle, steps):
+    """The 64-scene batched ensemble (2k-60k particles each, five material models)."""
+    import numpy as np
+    import xpbi
+    from xpbi import scenes
+    sl = scenes.config_c4()
+    sim = xpbi.Simulation(sl, device=0)
+    for q, sc in enumerate(sl):
+        sim.writeParams(xpbi.params_from_scene(sc), scene=q)
+    sim.step(settle, I, False)
+    blocks, tune_ms = sim.autotune(I, False, substeps=2)
+    eager_ms = sim.time_substeps(4, I, False) / 4
+    sim.graph_step(2, I, False)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.graph_step(steps, I, False)
+    sim.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    sizes = [si.N for si in sim.scenes_info]
+    kinds = np.bincount([sc["materials"][0]["kind"] for sc in sl], minlength=7)
+    out = dict(scenes=len(sl), particles=sim.N, smallest=min(sizes), largest=max(sizes),
+               materials=dict(DP=int(kinds[1]), NACC=int(kinds[2]), VM=int(kinds[3]), HB=int(kinds[4]),
+                              SNOW=int(kinds[5])),
+               iterations=I, settle_substeps=settle, steps=steps, launch_blocks=blocks,
+               launch="one block per 256-particle run" if blocks == 0 else f"persistent grid of {blocks} blocks",
+               work_entries=sim.get_launch()[1], ms_per_substep_graph=round(ms, 4),
+               ms_per_substep_eager=round(eager_ms, 4), particle_substeps_per_s=round(sim.N / (ms / 1e3), 1))
+    sim.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +154,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-residual", action="store_true")
     ap.add_argument("--config", default="headline", choices=["headline", "c2"])
+    ap.add_argument("--no-ensemble", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the substep engine has no multi-GPU path in this build "
@@ -136,13 +171,20 @@ def main():
     print(f"[bench] {N} particles, {sim.numCells} cells; settling {args.settle} substeps", file=sys.stderr,
           flush=True)
     sim.step(args.settle, I, False)
-    sim.step(args.warmup, I, False)
+    blocks, tune_ms = sim.autotune(I, False, substeps=2)
+    print(f"[bench] autotuned launch: {blocks} blocks (0 = one per run), {tune_ms:.3f} ms/substep",
+          file=sys.stderr, flush=True)
+    sim.graph_step(args.warmup, I, False)
     sim.sync()
     print("[bench] timed region", file=sys.stderr, flush=True)
-    total_ms, per = sim.time_substeps_detail(args.steps, I, False)
-    ms_per_step = total_ms / args.steps
-    value = N * args.steps / (total_ms / 1e3)
-
+    t0 = time.perf_counter()
+    sim.graph_step(args.steps, I, False)
+    sim.sync()
+    ms_per_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    value = N / (ms_per_step / 1e3)
+    # per-kernel HIP-event times of an eager run of the same substep (a separate pass)
+    detail_steps = 10
+    total_ms, per = sim.time_substeps_detail(detail_steps, I, False)
     kern = {}
     for name, (ms, cnt) in per.items():
         avg = ms / max(cnt, 1)
@@ -162,7 +204,7 @@ def main():
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel="solveA",
                     alg_bytes_per_launch=ALG_BYTES["solveA"] * N)
     g_ms = sum(per[k][0] for k in GATHER if k in per)
-    g_bytes = (264 + 260 * I) * N * args.steps
+    g_bytes = (264 + 260 * I) * N * detail_steps
     gather = dict(achieved_GBs=round(g_bytes / (g_ms / 1e3) / 1e9, 1),
                   frac=round(g_bytes / (g_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4))
 
@@ -181,6 +223,12 @@ def main():
         threads = min(os.cpu_count() or 1, 16)
         print(f"[bench] CPU oracle baseline on {threads} threads", file=sys.stderr, flush=True)
         cpu = cpu_baseline(sim, scene, I, threads)
+    sim.close()
+
+    ens = None
+    if not args.no_ensemble:
+        print("[bench] 64-scene batched ensemble", file=sys.stderr, flush=True)
+        ens = ensemble(I, 60, 20)
 
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "particle-substeps/s", "n_gpus": 1,
@@ -189,13 +237,15 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"DP sand column 80x160x80 r=0.005 ({N} particles), plain Jacobi"
                    if args.config == "headline" else f"DP sand column 40x80x40 r=0.01 ({N} particles)",
-                   "particles": N, "iterations": I, "dt": scene["dt"], "settle_substeps": args.settle},
+                   "particles": N, "iterations": I, "dt": scene["dt"], "settle_substeps": args.settle,
+                   "timing": "HIP-graph replay of two captured substeps, autotuned launch shape",
+                   "launch_blocks": blocks},
+        "eager_ms_per_step": round(total_ms / detail_steps, 4),
         "roofline": roofline, "gather_kernels_roofline": gather, "kernels": kern,
         "stream_copy_GBs": round(copy_GBs, 1),
-        "jacobi_to_residual": resid, "cpu_baseline": cpu,
+        "jacobi_to_residual": resid, "batched_ensemble": ens, "cpu_baseline": cpu,
     }
     print(json.dumps(out))
-    sim.close()
 
 
 if __name__ == "__main__":

@@ -137,16 +137,54 @@ void xpbi_destroy(xpbi_sim* sim);
 const char* xpbi_last_error(const xpbi_sim* sim);
 
 /* Grid as computed at create (the JavaScript host packs it into Params the
- * way web/js/sim.js:174-176 does). */
+ * way web/js/sim.js:174-176 does); scene 0 of a batch. */
 int xpbi_grid_info(const xpbi_sim* sim, float origin[3], float* cell_size, int32_t dims[3], uint32_t* num_cells,
                    uint32_t* n_particles);
 
-/* writeMaterials(materials): web/js/sim.js:84-99; n <= 8 records. */
+/* writeMaterials(materials): web/js/sim.js:84-99; n <= 8 records (scene 0). */
 int xpbi_write_materials(xpbi_sim* sim, const xpbi_material* mats, int n);
 
 /* writeParams(p): web/js/sim.js:159-186.  The grid words and N must equal the
- * sim's own (xpbi_grid_info); support must not exceed cellSize. */
+ * sim's own (xpbi_grid_info); support must not exceed cellSize (scene 0). */
 int xpbi_write_params(xpbi_sim* sim, const xpbi_params* p);
+
+/* ---- Batches (no reference counterpart: the reference runs one Simulation per
+ * scene, web/js/sim.js:9-30).  `count` independent scenes of any sizes share one
+ * simulation: their particles are concatenated in scene order (scene q holds
+ * particles first_particle .. first_particle + n_particles - 1 of every field),
+ * their cell grids are stacked in one cell table (CELLSTART has the sum of the
+ * scenes' cells; PERM indexes the concatenated arrays), and every kernel of a
+ * substep is one launch for all scenes.  Scenes never interact.  Each scene
+ * has its own materials and params, written with the scene variants below; the
+ * Jacobi residual of xpbi_set_tolerance is taken over the whole batch.  A single
+ * scene (xpbi_create) is a batch of one. */
+int xpbi_create_batch(const xpbi_scene_desc* descs, int count, xpbi_sim** out);
+int xpbi_scene_count(const xpbi_sim* sim, int* count);
+int xpbi_scene_info(const xpbi_sim* sim, int scene, float origin[3], float* cell_size, int32_t dims[3],
+                    uint32_t* num_cells, uint32_t* n_particles, uint32_t* first_particle, uint32_t* first_cell);
+int xpbi_write_scene_materials(xpbi_sim* sim, int scene, const xpbi_material* mats, int n);
+int xpbi_write_scene_params(xpbi_sim* sim, int scene, const xpbi_params* p);
+
+/* Launch shape of the particle kernels (no reference counterpart).  The
+ * particles are cut into runs of 256 that never cross a scene (the work
+ * entries); blocks = 0 launches one block per entry, blocks > 0 a persistent
+ * grid of that many blocks (rounded up to a multiple of 8, at most one per
+ * entry) in which each XCD walks a contiguous eighth of the entries. */
+int xpbi_set_launch(xpbi_sim* sim, int blocks);
+int xpbi_get_launch(const xpbi_sim* sim, int* blocks, int* work_entries);
+
+/* substeps x substep like xpbi_step, replayed from a HIP graph of two captured
+ * substeps (captured on first use and again whenever the iteration count, water
+ * flag, tolerance, warm-start mode, launch shape or residual monitor changes;
+ * host polling of xpbi_set_tolerance is off inside the graph).  An odd
+ * remainder runs as an ordinary substep.  Results equal xpbi_step's. */
+int xpbi_graph_step(xpbi_sim* sim, int substeps, int iterations, int has_water);
+
+/* Autotuner: times graph-replayed substeps under each launch shape (one block
+ * per entry; persistent grids of 1, 2 and 4 blocks per compute unit), keeps the
+ * fastest, and restores the particle state it started from.  best_blocks gets
+ * the chosen xpbi_set_launch value, best_ms its time per substep. */
+int xpbi_autotune(xpbi_sim* sim, int iterations, int has_water, int substeps, int* best_blocks, float* best_ms);
 
 /* encodeSubstep(enc, iterations, hasWater): web/js/sim.js:220-272. */
 int xpbi_substep(xpbi_sim* sim, int iterations, int has_water);

@@ -36,8 +36,16 @@ template <class T> hipError_t dalloc(Sim* s, T** p, size_t count) {
   return hipSuccess;
 }
 
+void drop_graph(Sim* s) {
+  if (s->graphExec) hipGraphExecDestroy(s->graphExec);
+  if (s->graph) hipGraphDestroy(s->graph);
+  s->graphExec = nullptr;
+  s->graph = nullptr;
+}
+
 void free_all(Sim* s) {
   if (s->stream) hipStreamSynchronize(s->stream);
+  drop_graph(s);
   for (void* p : s->allocs) hipFree(p);
   s->allocs.clear();
   if (s->ctlHost) hipHostFree(s->ctlHost);
@@ -51,8 +59,8 @@ void free_all(Sim* s) {
 // derived constants, formed in f32 with no contraction so that they equal the
 // per-pair expressions of common.js evaluated in f32
 #pragma clang fp contract(off)
-void build_kp(Sim* s) {
-  const xpbi_params& p = s->params;
+void build_kp(Scene& sc) {
+  const xpbi_params& p = sc.params;
   KP k{};
   k.ox = p.gridOrigin[0]; k.oy = p.gridOrigin[1]; k.oz = p.gridOrigin[2]; k.cell = p.cellSize;
   k.gx = p.gridDims[0]; k.gy = p.gridDims[1]; k.gz = p.gridDims[2]; k.C = p.numCells;
@@ -83,7 +91,10 @@ void build_kp(Sim* s) {
   k.invH = 1.0f / h;
   k.halfInvH = 0.5f / h;
   k.gradC = -5.0f * k.sigW / (h * h);
-  s->kp = k;
+  k.cellBase = sc.cellBase;
+  k.pBase = sc.pBase;
+  k.pEnd = sc.pBase + sc.N;
+  sc.kp = k;
 }
 #pragma clang fp contract(on)
 
@@ -94,8 +105,10 @@ int check_launch(Sim* s, const char* what) {
 }
 
 int run_substep(Sim* s, int iterations, int has_water, int stop) {
-  if (!s->haveParams) return fail(s, XPBI_ERR_STATE, "substep before xpbi_write_params");
-  if (!s->haveMats) return fail(s, XPBI_ERR_STATE, "substep before xpbi_write_materials");
+  for (const Scene& sc : s->sc) {
+    if (!sc.haveParams) return fail(s, XPBI_ERR_STATE, "substep before xpbi_write_params");
+    if (!sc.haveMats) return fail(s, XPBI_ERR_STATE, "substep before xpbi_write_materials");
+  }
   if (iterations < 0) return fail(s, XPBI_ERR_ARG, "iterations < 0");
   if (stop < XPBI_STAGE_SORT || stop > XPBI_STAGE_FULL) return fail(s, XPBI_ERR_ARG, "bad stage");
   hipSetDevice(s->dev);
@@ -165,36 +178,57 @@ int h2d(Sim* s, void* dst, const void* src, size_t bytes) {
 
 extern "C" {
 
-int xpbi_create(const xpbi_scene_desc* d, xpbi_sim** out) {
+int xpbi_create(const xpbi_scene_desc* d, xpbi_sim** out) { return xpbi_create_batch(d, 1, out); }
+
+int xpbi_create_batch(const xpbi_scene_desc* descs, int count, xpbi_sim** out) {
   if (!out) return fail(nullptr, XPBI_ERR_ARG, "out is null");
   *out = nullptr;
-  if (!d) return fail(nullptr, XPBI_ERR_ARG, "desc is null");
-  if (d->N > 0 && !d->positions) return fail(nullptr, XPBI_ERR_ARG, "positions is null");
-  if (d->N > 0 && !d->tags && !d->matIds) return fail(nullptr, XPBI_ERR_ARG, "tags and matIds are both null");
-  if (!(d->h > 0.0) || !std::isfinite(d->h)) return fail(nullptr, XPBI_ERR_ARG, "h must be > 0");
-  // grid: sim.js:20-26 (double arithmetic, as in JavaScript)
-  const double cs = 2.0 * d->h;
-  const int pad = 2;
-  double dimsD[3];
-  for (int k = 0; k < 3; k++) {
-    if (!(d->domainMax[k] >= d->domainMin[k])) return fail(nullptr, XPBI_ERR_ARG, "domainMax < domainMin");
-    dimsD[k] = std::ceil((d->domainMax[k] - d->domainMin[k]) / cs) + 2 * pad;
+  if (!descs) return fail(nullptr, XPBI_ERR_ARG, "desc is null");
+  if (count < 1 || count > 65535) return fail(nullptr, XPBI_ERR_ARG, "scene count must be 1..65535");
+  std::vector<Scene> scs(count);
+  u64 Nt = 0, Ct = 0, nbt = 0;
+  for (int q = 0; q < count; q++) {
+    const xpbi_scene_desc* d = &descs[q];
+    if (d->N > 0 && !d->positions) return fail(nullptr, XPBI_ERR_ARG, "positions is null");
+    if (d->N > 0 && !d->tags && !d->matIds) return fail(nullptr, XPBI_ERR_ARG, "tags and matIds are both null");
+    if (!(d->h > 0.0) || !std::isfinite(d->h)) return fail(nullptr, XPBI_ERR_ARG, "h must be > 0");
+    if (d->device != descs[0].device) return fail(nullptr, XPBI_ERR_ARG, "the scenes of a batch share one device");
+    // grid: sim.js:20-26 (double arithmetic, as in JavaScript)
+    const double cs = 2.0 * d->h;
+    const int pad = 2;
+    double dimsD[3];
+    for (int k = 0; k < 3; k++) {
+      if (!(d->domainMax[k] >= d->domainMin[k])) return fail(nullptr, XPBI_ERR_ARG, "domainMax < domainMin");
+      dimsD[k] = std::ceil((d->domainMax[k] - d->domainMin[k]) / cs) + 2 * pad;
+    }
+    const double cells = dimsD[0] * dimsD[1] * dimsD[2];
+    if (cells >= 4294967295.0 || dimsD[0] * dimsD[1] >= 2147483647.0)
+      return fail(nullptr, XPBI_ERR_ARG, "grid too large");
+    Scene& sc = scs[q];
+    sc.N = d->N;
+    sc.h = d->h;
+    for (int k = 0; k < 3; k++) {
+      sc.dims[k] = (int)dimsD[k];
+      sc.origin[k] = (float)(d->domainMin[k] - pad * cs);
+    }
+    sc.cell = (float)cs;
+    sc.C = (u32)cells;
+    sc.pBase = (u32)Nt;
+    sc.cellBase = (u32)Ct;
+    Nt += sc.N;
+    Ct += sc.C;
+    nbt += (sc.N + 255) / 256;
   }
-  double cells = dimsD[0] * dimsD[1] * dimsD[2];
-  if (cells >= 4294967295.0 || dimsD[0] * dimsD[1] >= 2147483647.0)
-    return fail(nullptr, XPBI_ERR_ARG, "grid too large");
+  if (Nt >= (1ull << 31) || Ct >= 4294967295ull) return fail(nullptr, XPBI_ERR_ARG, "batch too large");
 
   xpbi_sim* h = new xpbi_sim();
   Sim* s = &h->s;
-  s->dev = d->device;
-  s->N = d->N;
-  s->h = d->h;
-  for (int k = 0; k < 3; k++) {
-    s->dims[k] = (int)dimsD[k];
-    s->origin[k] = (float)(d->domainMin[k] - pad * cs);
-  }
-  s->cell = (float)cs;
-  s->C = (u32)cells;
+  s->dev = descs[0].device;
+  s->sc = scs;
+  s->N = (u32)Nt;
+  s->C = (u32)Ct;
+  s->nb = (u32)nbt;
+  s->grid = s->nb;
 
   hipError_t e = hipSetDevice(s->dev);
   if (e != hipSuccess) { int r = hip_fail(nullptr, e, "hipSetDevice"); delete h; return r; }
@@ -202,7 +236,7 @@ int xpbi_create(const xpbi_scene_desc* d, xpbi_sim** out) {
   if (e != hipSuccess) { int r = hip_fail(nullptr, e, "hipStreamCreate"); delete h; return r; }
 
   const u32 N = s->N, C = s->C;
-  const u32 gN = (N + 255) / 256;
+  const u32 gN = s->nb;
   bool ok = true;
   auto A = [&](hipError_t r) { if (r != hipSuccess) ok = false; };
   for (int k = 0; k < 2; k++) {
@@ -232,7 +266,9 @@ int xpbi_create(const xpbi_scene_desc* d, xpbi_sim** out) {
   size_t np2 = scan_partials_needed(nr);
   A(dalloc(s, &s->scanPart, np > np2 ? np : np2));
   A(dalloc(s, &s->radixHist, nr > 0 ? nr : 1));
-  A(dalloc(s, &s->dMats, 8));
+  A(dalloc(s, &s->dMats, 8ull * count));
+  A(dalloc(s, &s->dKP, (size_t)count));
+  A(dalloc(s, &s->dTab, gN));
   A(dalloc(s, &s->residPart, 2ull * (gN > 0 ? gN : 1)));
   A(dalloc(s, &s->residOut, 2ull * kMaxResidIters));
   A(dalloc(s, &s->ctl, 1));
@@ -245,26 +281,34 @@ int xpbi_create(const xpbi_scene_desc* d, xpbi_sim** out) {
   s->perm = s->vals[0];
   s->sortedKeys = s->keys[0];
 
-  // initial state: sim.js:52-81
-  const float V0 = (float)(d->h * d->h * d->h);
+  // initial state: sim.js:52-81, scene after scene
   std::vector<float> pos(4ull * N), vel(4ull * N, 0.0f), aux(4ull * N, 0.0f), Fa(4ull * N, 0.0f),
       Fb(4ull * N, 0.0f), Fc(N, 1.0f);
   std::vector<u32> id(N), zeros((size_t)C + 1, 0u);
-  for (u32 i = 0; i < N; i++) {
-    pos[4ull * i + 0] = d->positions[3ull * i + 0];
-    pos[4ull * i + 1] = d->positions[3ull * i + 1];
-    pos[4ull * i + 2] = d->positions[3ull * i + 2];
-    pos[4ull * i + 3] = d->tags ? d->tags[i] : (float)d->matIds[i];
-    if (d->velocities) {
-      vel[4ull * i + 0] = d->velocities[3ull * i + 0];
-      vel[4ull * i + 1] = d->velocities[3ull * i + 1];
-      vel[4ull * i + 2] = d->velocities[3ull * i + 2];
+  std::vector<uint2> tab;
+  tab.reserve(gN);
+  for (int q = 0; q < count; q++) {
+    const xpbi_scene_desc* d = &descs[q];
+    const Scene& sc = s->sc[q];
+    const float V0 = (float)(d->h * d->h * d->h);
+    for (u32 j = 0; j < sc.N; j++) {
+      const u64 i = sc.pBase + j;
+      pos[4 * i + 0] = d->positions[3ull * j + 0];
+      pos[4 * i + 1] = d->positions[3ull * j + 1];
+      pos[4 * i + 2] = d->positions[3ull * j + 2];
+      pos[4 * i + 3] = d->tags ? d->tags[j] : (float)d->matIds[j];
+      if (d->velocities) {
+        vel[4 * i + 0] = d->velocities[3ull * j + 0];
+        vel[4 * i + 1] = d->velocities[3ull * j + 1];
+        vel[4 * i + 2] = d->velocities[3ull * j + 2];
+      }
+      aux[4 * i + 0] = V0;
+      aux[4 * i + 2] = d->states ? d->states[j] : 0.0f;
+      Fa[4 * i + 0] = 1.0f;  // c0.x
+      Fb[4 * i + 0] = 1.0f;  // c1.y
+      id[i] = (u32)i;
     }
-    aux[4ull * i + 0] = V0;
-    aux[4ull * i + 2] = d->states ? d->states[i] : 0.0f;
-    Fa[4ull * i + 0] = 1.0f;  // c0.x
-    Fb[4ull * i + 0] = 1.0f;  // c1.y
-    id[i] = i;
+    for (u32 f = 0; f < sc.N; f += 256) tab.push_back(make_uint2((u32)q, sc.pBase + f));
   }
   int r = XPBI_OK;
   if (N > 0) {
@@ -276,6 +320,7 @@ int xpbi_create(const xpbi_scene_desc* d, xpbi_sim** out) {
     if (!r) r = h2d(s, s->set[0].Fc, Fc.data(), 4ull * N);
     if (!r) r = h2d(s, s->set[0].id, id.data(), 4ull * N);
     if (!r) r = h2d(s, s->vals[0], id.data(), 4ull * N);
+    if (!r) r = h2d(s, s->dTab, tab.data(), sizeof(uint2) * tab.size());
   }
   if (!r) r = h2d(s, s->cellStart, zeros.data(), 4ull * ((size_t)C + 1));
   if (r) {
@@ -303,20 +348,41 @@ const char* xpbi_last_error(const xpbi_sim* h) { return h ? h->s.err.c_str() : g
 int xpbi_grid_info(const xpbi_sim* h, float origin[3], float* cell_size, int32_t dims[3], uint32_t* num_cells,
                    uint32_t* n_particles) {
   if (!h) return XPBI_ERR_ARG;
+  return xpbi_scene_info(h, 0, origin, cell_size, dims, num_cells, n_particles, nullptr, nullptr);
+}
+
+int xpbi_scene_count(const xpbi_sim* h, int* count) {
+  if (!h || !count) return XPBI_ERR_ARG;
+  *count = (int)h->s.sc.size();
+  return XPBI_OK;
+}
+
+int xpbi_scene_info(const xpbi_sim* h, int scene, float origin[3], float* cell_size, int32_t dims[3],
+                    uint32_t* num_cells, uint32_t* n_particles, uint32_t* first_particle, uint32_t* first_cell) {
+  if (!h) return XPBI_ERR_ARG;
   const Sim& s = h->s;
+  if (scene < 0 || scene >= (int)s.sc.size()) return XPBI_ERR_ARG;
+  const Scene& sc = s.sc[scene];
   for (int k = 0; k < 3; k++) {
-    if (origin) origin[k] = s.origin[k];
-    if (dims) dims[k] = s.dims[k];
+    if (origin) origin[k] = sc.origin[k];
+    if (dims) dims[k] = sc.dims[k];
   }
-  if (cell_size) *cell_size = s.cell;
-  if (num_cells) *num_cells = s.C;
-  if (n_particles) *n_particles = s.N;
+  if (cell_size) *cell_size = sc.cell;
+  if (num_cells) *num_cells = sc.C;
+  if (n_particles) *n_particles = sc.N;
+  if (first_particle) *first_particle = sc.pBase;
+  if (first_cell) *first_cell = sc.cellBase;
   return XPBI_OK;
 }
 
 int xpbi_write_materials(xpbi_sim* h, const xpbi_material* mats, int n) {
+  return xpbi_write_scene_materials(h, 0, mats, n);
+}
+
+int xpbi_write_scene_materials(xpbi_sim* h, int scene, const xpbi_material* mats, int n) {
   if (!h) return XPBI_ERR_ARG;
   Sim* s = &h->s;
+  if (scene < 0 || scene >= (int)s->sc.size()) return fail(s, XPBI_ERR_ARG, "no such scene");
   if (n < 0 || n > 8) return fail(s, XPBI_ERR_ARG, "material count must be 0..8");
   if (n > 0 && !mats) return fail(s, XPBI_ERR_ARG, "mats is null");
   xpbi_material m[8];
@@ -325,29 +391,39 @@ int xpbi_write_materials(xpbi_sim* h, const xpbi_material* mats, int n) {
   hipSetDevice(s->dev);
   hipError_t e = hipStreamSynchronize(s->stream);
   if (e != hipSuccess) return hip_fail(s, e, "hipStreamSynchronize");
-  int r = h2d(s, s->dMats, m, sizeof(m));
+  int r = h2d(s, s->dMats + 8 * scene, m, sizeof(m));
   if (r) return r;
-  std::memcpy(s->mats, m, sizeof(m));
-  s->haveMats = true;
+  Scene& sc = s->sc[scene];
+  std::memcpy(sc.mats, m, sizeof(m));
+  sc.haveMats = true;
   return XPBI_OK;
 }
 
-int xpbi_write_params(xpbi_sim* h, const xpbi_params* p) {
+int xpbi_write_params(xpbi_sim* h, const xpbi_params* p) { return xpbi_write_scene_params(h, 0, p); }
+
+int xpbi_write_scene_params(xpbi_sim* h, int scene, const xpbi_params* p) {
   if (!h) return XPBI_ERR_ARG;
   Sim* s = &h->s;
+  if (scene < 0 || scene >= (int)s->sc.size()) return fail(s, XPBI_ERR_ARG, "no such scene");
   if (!p) return fail(s, XPBI_ERR_ARG, "params is null");
+  Scene& sc = s->sc[scene];
   for (int k = 0; k < 3; k++) {
-    if (p->gridOrigin[k] != s->origin[k] || p->gridDims[k] != s->dims[k])
+    if (p->gridOrigin[k] != sc.origin[k] || p->gridDims[k] != sc.dims[k])
       return fail(s, XPBI_ERR_ARG, "params grid does not match the simulation grid");
   }
-  if (p->cellSize != s->cell || p->numCells != s->C) return fail(s, XPBI_ERR_ARG, "params grid does not match");
-  if (p->N != s->N) return fail(s, XPBI_ERR_ARG, "params N does not match the particle count");
+  if (p->cellSize != sc.cell || p->numCells != sc.C) return fail(s, XPBI_ERR_ARG, "params grid does not match");
+  if (p->N != sc.N) return fail(s, XPBI_ERR_ARG, "params N does not match the particle count");
   if (!(p->dt > 0.0f) || !(p->h > 0.0f)) return fail(s, XPBI_ERR_ARG, "dt and h must be > 0");
   if (!(p->support > 0.0f) || p->support > p->cellSize)
     return fail(s, XPBI_ERR_ARG, "support must be in (0, cellSize]");
-  s->params = *p;
-  build_kp(s);
-  s->haveParams = true;
+  hipSetDevice(s->dev);
+  hipError_t e = hipStreamSynchronize(s->stream);
+  if (e != hipSuccess) return hip_fail(s, e, "hipStreamSynchronize");
+  sc.params = *p;
+  build_kp(sc);
+  int r = h2d(s, s->dKP + scene, &sc.kp, sizeof(KP));
+  if (r) return r;
+  sc.haveParams = true;
   return XPBI_OK;
 }
 
@@ -686,6 +762,154 @@ int xpbi_collect_warmstart_pairs(xpbi_sim* h, int iterations, int has_water, flo
     memcpy(x + i * XPBI_MLP_INPUTS, &f[16 * i], XPBI_MLP_INPUTS * sizeof(float));
     if (w) w[i] = f[16 * i + 15] * f[16 * i + 15];
   }
+  return XPBI_OK;
+}
+
+int xpbi_set_launch(xpbi_sim* h, int blocks) {
+  if (!h) return XPBI_ERR_ARG;
+  Sim* s = &h->s;
+  if (blocks < 0) return fail(s, XPBI_ERR_ARG, "blocks must be >= 0");
+  u32 g = s->nb;
+  if (blocks > 0) {
+    g = ((u32)blocks + 7u) & ~7u;
+    if (g >= s->nb) g = s->nb;
+  }
+  s->grid = g > 0 ? g : 1;
+  s->persist = g < s->nb;
+  return XPBI_OK;
+}
+
+int xpbi_get_launch(const xpbi_sim* h, int* blocks, int* work_entries) {
+  if (!h) return XPBI_ERR_ARG;
+  if (blocks) *blocks = h->s.persist ? (int)h->s.grid : 0;
+  if (work_entries) *work_entries = (int)h->s.nb;
+  return XPBI_OK;
+}
+
+int xpbi_graph_step(xpbi_sim* h, int substeps, int iterations, int has_water) {
+  if (!h) return XPBI_ERR_ARG;
+  Sim* s = &h->s;
+  if (substeps < 0) return fail(s, XPBI_ERR_ARG, "substeps < 0");
+  if (substeps == 0 || s->N == 0) return XPBI_OK;
+  hipSetDevice(s->dev);
+  // the graph bakes in the buffers, the loop shape and the launch shape
+  float tol = s->tol;
+  long long tolBits = 0;
+  std::memcpy(&tolBits, &tol, sizeof(float));
+  const long long key[8] = {1, iterations, has_water != 0, tolBits, s->wsMode, (long long)s->grid,
+                            (long long)s->cur * 2 + s->vcur, s->residOn};
+  if (!s->graphExec || std::memcmp(key, s->graphKey, sizeof(key)) != 0) {
+    int r = xpbi_sync(h);
+    if (r) return r;
+    drop_graph(s);
+    const int ce = s->checkEvery;
+    const bool prof = s->profiling;
+    s->checkEvery = 0;  // no host polling inside a graph
+    s->profiling = false;
+    hipError_t e = hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+      s->checkEvery = ce;
+      s->profiling = prof;
+      return hip_fail(s, e, "hipStreamBeginCapture");
+    }
+    // two substeps return the set and velocity ping-pong to where they started
+    r = run_substep(s, iterations, has_water, XPBI_STAGE_FULL);
+    if (!r) r = run_substep(s, iterations, has_water, XPBI_STAGE_FULL);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(s->stream, &g);
+    s->checkEvery = ce;
+    s->profiling = prof;
+    if (r) {
+      if (g) hipGraphDestroy(g);
+      return r;
+    }
+    if (e != hipSuccess) return hip_fail(s, e, "hipStreamEndCapture");
+    s->graph = g;
+    e = hipGraphInstantiate(&s->graphExec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      drop_graph(s);
+      return hip_fail(s, e, "hipGraphInstantiate");
+    }
+    std::memcpy(s->graphKey, key, sizeof(key));
+  }
+  for (int k = 0; k + 1 < substeps; k += 2) {
+    hipError_t e = hipGraphLaunch(s->graphExec, s->stream);
+    if (e != hipSuccess) return hip_fail(s, e, "hipGraphLaunch");
+  }
+  if (substeps & 1) return run_substep(s, iterations, has_water, XPBI_STAGE_FULL);
+  return XPBI_OK;
+}
+
+// ---- autotuner: the launch shape of the particle kernels, timed on this simulation
+int xpbi_autotune(xpbi_sim* h, int iterations, int has_water, int substeps, int* best_blocks, float* best_ms) {
+  if (!h) return XPBI_ERR_ARG;
+  Sim* s = &h->s;
+  if (substeps < 1) return fail(s, XPBI_ERR_ARG, "substeps must be >= 1");
+  if (s->N == 0) return XPBI_OK;
+  int r = xpbi_sync(h);
+  if (r) return r;
+  hipSetDevice(s->dev);
+  int cus = 256;
+  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->dev);
+  // the state the trials start from (and end on): current particle set and velocity
+  const u32 N = s->N;
+  PSet& S = s->set[s->cur];
+  std::vector<void*> save;
+  const std::pair<void*, size_t> fields[] = {{S.pos, 16ull * N}, {S.Fa, 16ull * N}, {S.Fb, 16ull * N},
+                                             {S.Fc, 4ull * N},   {S.aux, 16ull * N}, {S.id, 4ull * N},
+                                             {s->vel[s->vcur], 16ull * N}};
+  for (auto& f : fields) {
+    void* p = nullptr;
+    if (hipMalloc(&p, f.second) != hipSuccess) {
+      for (void* q : save) hipFree(q);
+      return fail(s, XPBI_ERR_NOMEM, "autotune: device allocation failed");
+    }
+    hipMemcpyAsync(p, f.first, f.second, hipMemcpyDeviceToDevice, s->stream);
+    save.push_back(p);
+  }
+  const int cur0 = s->cur, vcur0 = s->vcur;
+  const int oldBlocks = s->persist ? (int)s->grid : 0;
+  std::vector<int> cands = {0, cus, 2 * cus, 4 * cus};
+  int best = 0;
+  float bestMs = 1e30f;
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  for (int c : cands) {
+    if (c > 0 && (u32)c >= s->nb) continue;
+    xpbi_set_launch(h, c);
+    // restore, warm one substep (graph capture for this shape), then time
+    s->cur = cur0;
+    s->vcur = vcur0;
+    for (size_t k = 0; k < save.size(); k++) hipMemcpyAsync(fields[k].first, save[k], fields[k].second,
+                                                           hipMemcpyDeviceToDevice, s->stream);
+    if ((r = xpbi_graph_step(h, 2, iterations, has_water))) break;
+    hipEventRecord(e0, s->stream);
+    if ((r = xpbi_graph_step(h, 2 * substeps, iterations, has_water))) break;
+    hipEventRecord(e1, s->stream);
+    hipEventSynchronize(e1);
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, e0, e1);
+    ms /= (float)(2 * substeps);
+    if (ms < bestMs) {
+      bestMs = ms;
+      best = c;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  // the simulation continues from where it was, with the fastest shape
+  xpbi_sync(h);
+  s->cur = cur0;
+  s->vcur = vcur0;
+  for (size_t k = 0; k < save.size(); k++)
+    hipMemcpy(fields[k].first, save[k], fields[k].second, hipMemcpyDeviceToDevice);
+  for (void* q : save) hipFree(q);
+  drop_graph(s);
+  xpbi_set_launch(h, r ? oldBlocks : best);
+  if (r) return r;
+  if (best_blocks) *best_blocks = best;
+  if (best_ms) *best_ms = bestMs;
   return XPBI_OK;
 }
 

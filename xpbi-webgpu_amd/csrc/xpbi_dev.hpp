@@ -55,6 +55,9 @@ struct KP {
   float invH;    // 1 / h
   float halfInvH;  // 0.5 / h
   float gradC;   // -5 sigma / h^2: grad W = dxv * gradC * (1 - q/2)^3
+  // the scene's place in a batch: its cells are cellBase .. cellBase + C - 1 of
+  // the shared cell table, its particles pBase .. pEnd - 1 of the shared arrays
+  u32 cellBase, pBase, pEnd;
 };
 
 // ---------------------------------------------------------------- vectors
@@ -233,7 +236,7 @@ __device__ __forceinline__ i3 cellCoord(const KP& P, f3 p) {
   return i3{cx, cy, cz};
 }
 __device__ __forceinline__ u32 cellIndex(const KP& P, i3 c) {
-  return u32(c.x) + u32(c.y) * u32(P.gx) + u32(c.z) * P.gxy;
+  return P.cellBase + u32(c.x) + u32(c.y) * u32(P.gx) + u32(c.z) * P.gxy;
 }
 // pos.w holds the material index as a float value (simkernels.js:41-44); the
 // mask keeps a corrupt tag inside the 8-entry material table.
@@ -566,6 +569,31 @@ __device__ __forceinline__ u32 xcdBlock() {
   u32 nb = gridDim.x, b = blockIdx.x;
   u32 q = nb >> 3, r = nb & 7u, x = b & 7u, k = b >> 3;
   return (x < r) ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
+// ---------------------------------------------------------------- work table
+// A launch walks a table of block-sized particle runs that never cross a scene
+// boundary: entry t = (scene, first particle), the run ending at
+// min(first + 256, scene end).  A batch of scenes is one launch per kernel.
+struct WorkTab {
+  const uint2* __restrict__ tab;
+  u32 nb;
+};
+// fn(scene, first) for this block's entries.  kPersist = false: one entry per
+// block (the grid covers the table), XCD-contiguous.  kPersist = true: a
+// persistent grid (a multiple of 8 blocks) where XCD x walks the x-th
+// contiguous eighth of the table; a separate instantiation, because the loop
+// costs registers the one-entry form does not need.
+template <bool kPersist, class Fn>
+__device__ __forceinline__ void forWork(const WorkTab& W, Fn&& fn) {
+  if (!kPersist) {
+    const u32 t = xcdBlock();
+    if (t < W.nb) fn(W.tab[t].x, W.tab[t].y);
+    return;
+  }
+  const u32 x = blockIdx.x & 7u, per = (W.nb + 7u) >> 3, step = gridDim.x >> 3;
+  const u32 t1 = min(x * per + per, W.nb);
+  for (u32 t = x * per + (blockIdx.x >> 3); t < t1; t += step) fn(W.tab[t].x, W.tab[t].y);
 }
 
 }  // namespace xpbi

@@ -52,10 +52,10 @@ __device__ __forceinline__ bool skipIteration(const SolveCtl* ctl, u32 k) {
 constexpr int kMaxProfileMarks = 4096;
 constexpr int kMaxResidIters = 4096;
 
-struct Sim {
-  int dev = 0;
-  hipStream_t stream = nullptr;
+// one scene of a simulation (a batch holds several; a single scene is a batch of one)
+struct Scene {
   u32 N = 0, C = 0;
+  u32 pBase = 0, cellBase = 0;  // its particles / cells in the shared arrays and cell table
   int dims[3] = {0, 0, 0};
   float origin[3] = {0, 0, 0};
   float cell = 0;
@@ -64,7 +64,25 @@ struct Sim {
   xpbi_params params{};
   xpbi_material mats[8]{};
   KP kp{};
-  DevMat* dMats = nullptr;
+};
+
+struct Sim {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  u32 N = 0, C = 0;  // all scenes
+  std::vector<Scene> sc;
+  KP* dKP = nullptr;       // per scene
+  DevMat* dMats = nullptr;  // 8 per scene
+  uint2* dTab = nullptr;   // work table: (scene, first particle) per 256-particle run
+  u32 nb = 0;              // work table entries
+  u32 grid = 0;            // blocks of a particle launch (nb, or a persistent grid)
+  bool persist = false;    // grid < nb: the kernels' persistent instantiation
+
+  // captured two-substep graph and the configuration it was captured with
+  hipGraphExec_t graphExec = nullptr;
+  hipGraph_t graph = nullptr;
+  long long graphKey[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  WorkTab work() const { return WorkTab{dTab, nb}; }
 
   PSet set[2]{};
   int cur = 0;
@@ -124,6 +142,13 @@ struct Sim {
   std::vector<void*> allocs;
   std::string err;
 };
+
+// a work-table kernel in the instantiation that matches s->grid (s in scope)
+#define XPBI_LAUNCH_W(K, ...)                                   \
+  do {                                                          \
+    if (s->persist) hipLaunchKernelGGL(K<true>, __VA_ARGS__);   \
+    else hipLaunchKernelGGL(K<false>, __VA_ARGS__);             \
+  } while (0)
 
 // launchers
 void launch_sort(Sim* s);

@@ -20,7 +20,7 @@ __device__ __forceinline__ void forRows(const KP& P, const u32* __restrict__ cs,
     for (int dy = -1; dy <= 1; dy++) {
       int y = cc.y + dy;
       if (y < 0 || y >= P.gy) continue;
-      u32 row = (u32)y * (u32)P.gx + (u32)z * P.gxy;
+      u32 row = P.cellBase + (u32)y * (u32)P.gx + (u32)z * P.gxy;
       u32 beg = cs[row + x0];
       u32 end = min(cs[row + x1 + 1u], beg + kRowCap);
 #pragma unroll 1
@@ -128,6 +128,7 @@ struct MatLds {
 };
 __device__ __forceinline__ void loadMatLds(MatLds& t, const DevMat* __restrict__ mats) {
   const u32 tid = threadIdx.x;
+  __syncthreads();  // a previous scene's table may still be read
   if (tid < 128) ((float*)t.m)[tid] = ((const float*)mats)[tid];
   if (tid < 8) {
     t.mass[tid] = mats[tid].mass;
@@ -140,6 +141,13 @@ __device__ __forceinline__ void loadMatLds(MatLds& t, const DevMat* __restrict__
     t.solidMask = mask;
   }
   __syncthreads();
+}
+// the scene's 8 materials (mats holds 8 per scene), reloaded when the scene changes
+__device__ __forceinline__ void sceneMats(MatLds& t, const DevMat* __restrict__ mats, u32 scene, u32* cur) {
+  if (scene != *cur) {
+    loadMatLds(t, mats + 8 * scene);
+    *cur = scene;
+  }
 }
 __device__ __forceinline__ MatV matv(const MatLds& t, u32 k) {
   const DevMat& r = t.m[k];

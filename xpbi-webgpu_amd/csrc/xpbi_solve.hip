@@ -18,40 +18,45 @@ namespace xpbi {
 // simkernels.js:189-221.  A is accumulated in f32; the SVD pseudo-inverse with
 // absolute tolerance 0.2 runs in FP64.  The same candidate walk records the
 // support-neighbour list of the substep.
-__global__ __launch_bounds__(256) void k_computeL(KP P, const float4* __restrict__ pos, const float4* __restrict__ vin,
+template <bool kPersist>
+__global__ __launch_bounds__(256) void k_computeL(const KP* __restrict__ kps, WorkTab W, const float4* __restrict__ pos, const float4* __restrict__ vin,
                                                   const u32* __restrict__ cs, M3P L, u32* __restrict__ nIdx,
-                                                  u32* __restrict__ nCnt, u32 K) {
-  u32 i = xcdBlock() * 256u + threadIdx.x;
-  if (i >= P.N) return;
-  f3 xi = xyz(pos[i]);
-  f33 A = mzero<float>();
-  u32 n = 0;
-  forRows(P, cs, cellCoord(P, xi), [&](u32 b) {
-    if (b != i) {
-      f3 dxv = xi - xyz(pos[b]);
-      float r2 = dot(dxv, dxv);
-      if (r2 < P.supp2) {
-        f3 g = gradWfast(P, dxv, r2);
-        f3 r = -dxv;
-        A = madd(A, mscale(outer(g, r), vin[b].w));
-        if (n < K) nIdx[(size_t)n * P.N + i] = b;
-        n++;
+                                                  u32* __restrict__ nCnt, u32 K, u32 NT) {
+  forWork<kPersist>(W, [&](u32 scene, u32 first) {
+    const KP& P = kps[scene];
+    const u32 i = first + threadIdx.x;
+    if (i >= P.pEnd) return;
+    f3 xi = xyz(pos[i]);
+    f33 A = mzero<float>();
+    u32 n = 0;
+    forRows(P, cs, cellCoord(P, xi), [&](u32 b) {
+      if (b != i) {
+        f3 dxv = xi - xyz(pos[b]);
+        float r2 = dot(dxv, dxv);
+        if (r2 < P.supp2) {
+          f3 g = gradWfast(P, dxv, r2);
+          f3 r = -dxv;
+          A = madd(A, mscale(outer(g, r), vin[b].w));
+          if (n < K) nIdx[(size_t)n * NT + i] = b;
+          n++;
+        }
       }
-    }
+    });
+    nCnt[i] = n;
+    SVD3<double> s = svd3<double>(mcast<double>(A));
+    d3 inv = splat3(0.0);
+    if (fabs(s.sig.x) > 0.2) inv.x = 1.0 / s.sig.x;
+    if (fabs(s.sig.y) > 0.2) inv.y = 1.0 / s.sig.y;
+    if (fabs(s.sig.z) > 0.2) inv.z = 1.0 / s.sig.z;
+    stm(L, i, mcast<float>(diagMul(s.V, inv, s.U)));
   });
-  nCnt[i] = n;
-  SVD3<double> s = svd3<double>(mcast<double>(A));
-  d3 inv = splat3(0.0);
-  if (fabs(s.sig.x) > 0.2) inv.x = 1.0 / s.sig.x;
-  if (fabs(s.sig.y) > 0.2) inv.y = 1.0 / s.sig.y;
-  if (fabs(s.sig.z) > 0.2) inv.z = 1.0 / s.sig.z;
-  stm(L, i, mcast<float>(diagMul(s.V, inv, s.U)));
 }
 
 // ---------------------------------------------------------------- solveA
 // simkernels.js:248-309.  With part != nullptr the block also reduces the
 // squared Delta-lambda numerator (the residual) and C^2 over solid particles.
-__global__ __launch_bounds__(256) void k_solveA(KP P, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
+template <bool kPersist>
+__global__ __launch_bounds__(256) void k_solveA(const KP* __restrict__ kps, WorkTab W, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
                                                 const float4* __restrict__ vin, M3C Fm, M3C Lm,
                                                 float4* __restrict__ aux, M3P Sp, float4* __restrict__ selfDv,
                                                 const u32* __restrict__ cs, NLc nl, double* __restrict__ part,
@@ -59,10 +64,13 @@ __global__ __launch_bounds__(256) void k_solveA(KP P, const DevMat* __restrict__
                                                 const float* __restrict__ hbHint) {
   if (skipIteration(ctl, k)) return;
   __shared__ MatLds T;
-  loadMatLds(T, mats);
-  const u32 i = xcdBlock() * 256u + threadIdx.x;
+  u32 curScene = ~0u;
   float r2 = 0.0f, c2 = 0.0f;
-  if (i < P.N) {
+  forWork<kPersist>(W, [&](u32 scene, u32 first) {
+    sceneMats(T, mats, scene, &curScene);
+    const KP& P = kps[scene];
+    const u32 i = first + threadIdx.x;
+    if (i >= P.pEnd) return;
     const float4 pi = pos[i];
     const f3 vi = xyz(vin[i]);
     Moments mo;
@@ -71,7 +79,7 @@ __global__ __launch_bounds__(256) void k_solveA(KP P, const DevMat* __restrict__
     if (solid) mo = neighbourMoments<true>(P, cs, pos, vin, nl, T.invMass, T.solidMask, i, xyz(pi), vi);
     const float hint = (hbHint && T.m[mi].kind == MAT_HB) ? hbHint[i] : -1.0f;
     solveAFinish(P, T, i, pi, vi, solid, mo, Fm, Lm, aux, Sp, selfDv, &r2, &c2, hint);
-  }
+  });
   if (part) blockReduce2((double)r2, (double)c2, part + 2 * blockIdx.x);
 }
 
@@ -124,172 +132,201 @@ __global__ __launch_bounds__(256) void k_parity_fix(const SolveCtl* __restrict__
 }
 
 // simkernels.js:315-366
-__global__ __launch_bounds__(256) void k_solveB(KP P, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
+template <bool kPersist>
+__global__ __launch_bounds__(256) void k_solveB(const KP* __restrict__ kps, WorkTab W, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
                                                 const float4* __restrict__ vin, M3C Sp,
                                                 const float4* __restrict__ selfDv, const u32* __restrict__ cs,
                                                 NLc nl, float4* __restrict__ vout, const SolveCtl* __restrict__ ctl,
                                                 u32 k) {
   if (skipIteration(ctl, k)) return;
   __shared__ MatLds T;
-  loadMatLds(T, mats);
-  const u32 i = xcdBlock() * 256u + threadIdx.x;
-  if (i >= P.N) return;
-  const float4 pi = pos[i];
-  const float4 vi4 = vin[i];
-  const u32 mi = matOfTag(pi.w);
-  f3 dv = splat3(0.0f), corr = splat3(0.0f);
-  if ((T.solidMask >> mi) & 1u)
-    solveBGatherGlobal(P, T, i, pi, vi4, pos, vin, Sp, selfDv, cs, nl, &dv, &corr);
-  solveBFinish(P, T, i, pi, vi4, dv, corr, vout);
+  u32 curScene = ~0u;
+  forWork<kPersist>(W, [&](u32 scene, u32 first) {
+    sceneMats(T, mats, scene, &curScene);
+    const KP& P = kps[scene];
+    const u32 i = first + threadIdx.x;
+    if (i >= P.pEnd) return;
+    const float4 pi = pos[i];
+    const float4 vi4 = vin[i];
+    const u32 mi = matOfTag(pi.w);
+    f3 dv = splat3(0.0f), corr = splat3(0.0f);
+    if ((T.solidMask >> mi) & 1u)
+      solveBGatherGlobal(P, T, i, pi, vi4, pos, vin, Sp, selfDv, cs, nl, &dv, &corr);
+    solveBFinish(P, T, i, pi, vi4, dv, corr, vout);
+  });
 }
 
 // ---------------------------------------------------------------- PBF
 // simkernels.js:407-439
-__global__ __launch_bounds__(256) void k_pbfA(KP P, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
+template <bool kPersist>
+__global__ __launch_bounds__(256) void k_pbfA(const KP* __restrict__ kps, WorkTab W, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
                                               const float4* __restrict__ vin, float4* __restrict__ aux,
                                               const u32* __restrict__ cs, const SolveCtl* __restrict__ ctl, u32 k) {
   if (skipIteration(ctl, k)) return;
   __shared__ MatLds T;
-  loadMatLds(T, mats);
-  const u32 i = xcdBlock() * 256u + threadIdx.x;
-  if (i >= P.N) return;
-  const float4 pi = pos[i];
-  const u32 mi = matOfTag(pi.w);
-  if ((T.solidMask >> mi) & 1u) { aux[i].w = 0.0f; return; }
-  const float rho0 = T.m[mi].density;
-  const f3 xn = xyz(pi);
-  const f3 xci = xn + P.dt * xyz(vin[i]);
-  float rho = T.mass[mi] * P.poly6_0;
-  f3 gradSum = splat3(0.0f);
-  float grad2 = 0.0f;
-  forRows(P, cs, cellCoord(P, xn), [&](u32 b) {
-    if (b != i) {
-      float4 pb = pos[b];
-      f3 xcb = xyz(pb) + P.dt * xyz(vin[b]);
-      f3 dxv = xci - xcb;
-      if (dot(dxv, dxv) < P.supp2) {
-        float mb = T.mass[matOfTag(pb.w)];
-        rho = rho + mb * kernelPoly6(P, len(dxv));
-        f3 g = kernelSpikyGrad(P, dxv) * (mb / rho0);
-        gradSum = gradSum + g;
-        grad2 = grad2 + dot(g, g);
+  u32 curScene = ~0u;
+  forWork<kPersist>(W, [&](u32 scene, u32 first) {
+    sceneMats(T, mats, scene, &curScene);
+    const KP& P = kps[scene];
+    const u32 i = first + threadIdx.x;
+    if (i >= P.pEnd) return;
+    const float4 pi = pos[i];
+    const u32 mi = matOfTag(pi.w);
+    if ((T.solidMask >> mi) & 1u) { aux[i].w = 0.0f; return; }
+    const float rho0 = T.m[mi].density;
+    const f3 xn = xyz(pi);
+    const f3 xci = xn + P.dt * xyz(vin[i]);
+    float rho = T.mass[mi] * P.poly6_0;
+    f3 gradSum = splat3(0.0f);
+    float grad2 = 0.0f;
+    forRows(P, cs, cellCoord(P, xn), [&](u32 b) {
+      if (b != i) {
+        float4 pb = pos[b];
+        f3 xcb = xyz(pb) + P.dt * xyz(vin[b]);
+        f3 dxv = xci - xcb;
+        if (dot(dxv, dxv) < P.supp2) {
+          float mb = T.mass[matOfTag(pb.w)];
+          rho = rho + mb * kernelPoly6(P, len(dxv));
+          f3 g = kernelSpikyGrad(P, dxv) * (mb / rho0);
+          gradSum = gradSum + g;
+          grad2 = grad2 + dot(g, g);
+        }
       }
-    }
+    });
+    float C = wmax(rho / rho0 - 1.0f, 0.0f);
+    aux[i].w = -C / (grad2 + dot(gradSum, gradSum) + P.pbfEps);
   });
-  float C = wmax(rho / rho0 - 1.0f, 0.0f);
-  aux[i].w = -C / (grad2 + dot(gradSum, gradSum) + P.pbfEps);
 }
 
 // simkernels.js:441-477
-__global__ __launch_bounds__(256) void k_pbfB(KP P, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
+template <bool kPersist>
+__global__ __launch_bounds__(256) void k_pbfB(const KP* __restrict__ kps, WorkTab W, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
                                               const float4* __restrict__ vin, const float4* __restrict__ aux,
                                               const u32* __restrict__ cs, float4* __restrict__ vout,
                                               const SolveCtl* __restrict__ ctl, u32 k) {
   if (skipIteration(ctl, k)) return;
   __shared__ MatLds T;
-  loadMatLds(T, mats);
-  const u32 i = xcdBlock() * 256u + threadIdx.x;
-  if (i >= P.N) return;
-  const float4 pi = pos[i];
-  const u32 mi = matOfTag(pi.w);
-  const f3 xn = xyz(pi);
-  const float4 vi4 = vin[i];
-  const f3 xci = xn + P.dt * xyz(vi4);
-  f3 dp = splat3(0.0f);
-  const float li = aux[i].w;
-  const bool iAmWater = !((T.solidMask >> mi) & 1u);
-  forRows(P, cs, cellCoord(P, xn), [&](u32 b) {
-    if (b != i) {
-      float4 pb = pos[b];
-      u32 mb = matOfTag(pb.w);
-      bool bWater = !((T.solidMask >> mb) & 1u);
-      if (iAmWater || bWater) {
-        f3 xcb = xyz(pb) + P.dt * xyz(vin[b]);
-        f3 dxv = xci - xcb;
-        if (dot(dxv, dxv) < P.supp2) {
-          u32 wk = bWater ? mb : mi;
-          float lb = aux[b].w;
-          float w = kernelPoly6(P, len(dxv));
-          float ratio = w / wmax(P.w0, 1e-12f);
-          float r2 = ratio * ratio;
-          float sCorr = -P.sCorrK * (r2 * r2);
-          dp = dp + (li + lb + sCorr) * kernelSpikyGrad(P, dxv) * (T.m[wk].mass / T.m[wk].density);
+  u32 curScene = ~0u;
+  forWork<kPersist>(W, [&](u32 scene, u32 first) {
+    sceneMats(T, mats, scene, &curScene);
+    const KP& P = kps[scene];
+    const u32 i = first + threadIdx.x;
+    if (i >= P.pEnd) return;
+    const float4 pi = pos[i];
+    const u32 mi = matOfTag(pi.w);
+    const f3 xn = xyz(pi);
+    const float4 vi4 = vin[i];
+    const f3 xci = xn + P.dt * xyz(vi4);
+    f3 dp = splat3(0.0f);
+    const float li = aux[i].w;
+    const bool iAmWater = !((T.solidMask >> mi) & 1u);
+    forRows(P, cs, cellCoord(P, xn), [&](u32 b) {
+      if (b != i) {
+        float4 pb = pos[b];
+        u32 mb = matOfTag(pb.w);
+        bool bWater = !((T.solidMask >> mb) & 1u);
+        if (iAmWater || bWater) {
+          f3 xcb = xyz(pb) + P.dt * xyz(vin[b]);
+          f3 dxv = xci - xcb;
+          if (dot(dxv, dxv) < P.supp2) {
+            u32 wk = bWater ? mb : mi;
+            float lb = aux[b].w;
+            float w = kernelPoly6(P, len(dxv));
+            float ratio = w / wmax(P.w0, 1e-12f);
+            float r2 = ratio * ratio;
+            float sCorr = -P.sCorrK * (r2 * r2);
+            dp = dp + (li + lb + sCorr) * kernelSpikyGrad(P, dxv) * (T.m[wk].mass / T.m[wk].density);
+          }
         }
       }
-    }
+    });
+    float scale = iAmWater ? 1.0f : 0.5f;
+    f3 v = xyz(vi4) + dp * (scale / P.dt);
+    vout[i] = make_float4(v.x, v.y, v.z, vi4.w);
   });
-  float scale = iAmWater ? 1.0f : 0.5f;
-  f3 v = xyz(vi4) + dp * (scale / P.dt);
-  vout[i] = make_float4(v.x, v.y, v.z, vi4.w);
 }
 
 // ---------------------------------------------------------------- XSPH
 // simkernels.js:480-507
-__global__ __launch_bounds__(256) void k_xsph(KP P, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
+template <bool kPersist>
+__global__ __launch_bounds__(256) void k_xsph(const KP* __restrict__ kps, WorkTab W, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
                                               const float4* __restrict__ vin, const u32* __restrict__ cs, NLc nl,
                                               float4* __restrict__ vout) {
   __shared__ MatLds T;
-  loadMatLds(T, mats);
-  const u32 i = xcdBlock() * 256u + threadIdx.x;
-  if (i >= P.N) return;
-  const float4 pi = pos[i];
-  const f3 xi = xyz(pi);
-  const float4 vi4 = vin[i];
-  const f3 vi = xyz(vi4);
-  const bool iSolid = solidTag(T.solidMask, pi.w);
-  f3 acc = splat3(0.0f);
-  forSupportG(P, cs, nl, i, xi, [&](u32 b) { return PV{pos[b], vin[b]}; },
-              [&](u32 b, const PV& r, f3 dxv, float r2) {
-                if (solidTag(T.solidMask, r.p.w) == iSolid) acc = acc + r.v.w * (xyz(r.v) - vi) * kernelWfast(P, r2);
-              });
-  float c = P.xsphC;
-  if (!iSolid) c = P.xsphC * 10.0f;
-  f3 v = vi + c * acc;
-  vout[i] = make_float4(v.x, v.y, v.z, vi4.w);
+  u32 curScene = ~0u;
+  forWork<kPersist>(W, [&](u32 scene, u32 first) {
+    sceneMats(T, mats, scene, &curScene);
+    const KP& P = kps[scene];
+    const u32 i = first + threadIdx.x;
+    if (i >= P.pEnd) return;
+    const float4 pi = pos[i];
+    const f3 xi = xyz(pi);
+    const float4 vi4 = vin[i];
+    const f3 vi = xyz(vi4);
+    const bool iSolid = solidTag(T.solidMask, pi.w);
+    f3 acc = splat3(0.0f);
+    forSupportG(P, cs, nl, i, xi, [&](u32 b) { return PV{pos[b], vin[b]}; },
+                [&](u32 b, const PV& r, f3 dxv, float r2) {
+                  if (solidTag(T.solidMask, r.p.w) == iSolid) acc = acc + r.v.w * (xyz(r.v) - vi) * kernelWfast(P, r2);
+                });
+    float c = P.xsphC;
+    if (!iSolid) c = P.xsphC * 10.0f;
+    f3 v = vi + c * acc;
+    vout[i] = make_float4(v.x, v.y, v.z, vi4.w);
+  });
 }
 
 // ---------------------------------------------------------------- updateF
 // simkernels.js:510-525
-__global__ __launch_bounds__(256) void k_updateF(KP P, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
+template <bool kPersist>
+__global__ __launch_bounds__(256) void k_updateF(const KP* __restrict__ kps, WorkTab W, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
                                                  const float4* __restrict__ vin, M3P Fm, M3C Lm,
                                                  float4* __restrict__ aux, const u32* __restrict__ cs, NLc nl,
                                                  const float* __restrict__ hbHint) {
   __shared__ MatLds T;
-  loadMatLds(T, mats);
-  const u32 i = xcdBlock() * 256u + threadIdx.x;
-  if (i >= P.N) return;
-  const float4 pi = pos[i];
-  const u32 mi = matOfTag(pi.w);
-  if (!((T.solidMask >> mi) & 1u)) return;
-  const float state = aux[i].z;
-  MatV m = hardenedMat(matv(T, mi), state);
-  const f3 xi = xyz(pi);
-  const f33 Li = ldm(Lm, i);
-  const Moments mo = neighbourMoments<false>(P, cs, pos, vin, nl, T.invMass, T.solidMask, i, xi, xyz(vin[i]));
-  const f33 gradV = mul(mo.M, transpose(Li));
-  f33 Ftr = mul(madd(mident<float>(), mscale(gradV, P.dt)), ldm(Fm, i));
-  SVD3<float> sv = svd3_guarded(Ftr);
-  float st;
-  const float hint = (hbHint && m.kind == MAT_HB) ? hbHint[i] : -1.0f;
-  f3 sig = returnMap(sv.sig, m, state, P.dt, &st, hint);
-  stm(Fm, i, diagMul(sv.U, safeSig(sig), sv.V));
-  aux[i].z = st;
+  u32 curScene = ~0u;
+  forWork<kPersist>(W, [&](u32 scene, u32 first) {
+    sceneMats(T, mats, scene, &curScene);
+    const KP& P = kps[scene];
+    const u32 i = first + threadIdx.x;
+    if (i >= P.pEnd) return;
+    const float4 pi = pos[i];
+    const u32 mi = matOfTag(pi.w);
+    if (!((T.solidMask >> mi) & 1u)) return;
+    const float state = aux[i].z;
+    MatV m = hardenedMat(matv(T, mi), state);
+    const f3 xi = xyz(pi);
+    const f33 Li = ldm(Lm, i);
+    const Moments mo = neighbourMoments<false>(P, cs, pos, vin, nl, T.invMass, T.solidMask, i, xi, xyz(vin[i]));
+    const f33 gradV = mul(mo.M, transpose(Li));
+    f33 Ftr = mul(madd(mident<float>(), mscale(gradV, P.dt)), ldm(Fm, i));
+    SVD3<float> sv = svd3_guarded(Ftr);
+    float st;
+    const float hint = (hbHint && m.kind == MAT_HB) ? hbHint[i] : -1.0f;
+    f3 sig = returnMap(sv.sig, m, state, P.dt, &st, hint);
+    stm(Fm, i, diagMul(sv.U, safeSig(sig), sv.V));
+    aux[i].z = st;
+  });
 }
 
 // ---------------------------------------------------------------- integrate
 // simkernels.js:528-542
-__global__ __launch_bounds__(256) void k_integrate(KP P, float4* __restrict__ pos, float4* __restrict__ vin) {
-  const u32 i = xcdBlock() * 256u + threadIdx.x;
-  if (i >= P.N) return;
-  float4 v4 = vin[i];
-  f3 v = xyz(v4);
-  float vl = len(v);
-  if (vl > P.vmax) v = v * (P.vmax / vl);
-  vin[i] = make_float4(v.x, v.y, v.z, v4.w);
-  float4 p = pos[i];
-  f3 x = xyz(p) + P.dt * v;
-  x = f3{wclamp(x.x, P.dminx, P.dmaxx), wclamp(x.y, P.dminy, P.dmaxy), wclamp(x.z, P.dminz, P.dmaxz)};
-  pos[i] = make_float4(x.x, x.y, x.z, p.w);
+template <bool kPersist>
+__global__ __launch_bounds__(256) void k_integrate(const KP* __restrict__ kps, WorkTab W, float4* __restrict__ pos, float4* __restrict__ vin) {
+  forWork<kPersist>(W, [&](u32 scene, u32 first) {
+    const KP& P = kps[scene];
+    const u32 i = first + threadIdx.x;
+    if (i >= P.pEnd) return;
+    float4 v4 = vin[i];
+    f3 v = xyz(v4);
+    float vl = len(v);
+    if (vl > P.vmax) v = v * (P.vmax / vl);
+    vin[i] = make_float4(v.x, v.y, v.z, v4.w);
+    float4 p = pos[i];
+    f3 x = xyz(p) + P.dt * v;
+    x = f3{wclamp(x.x, P.dminx, P.dmaxx), wclamp(x.y, P.dminy, P.dmaxy), wclamp(x.z, P.dminz, P.dmaxz)};
+    pos[i] = make_float4(x.x, x.y, x.z, p.w);
+  });
 }
 
 // ---------------------------------------------------------------- learned warm start
@@ -306,17 +343,21 @@ __global__ __launch_bounds__(256) void k_integrate(KP P, float4* __restrict__ po
 // 13 HB root position of the return map (NaN: none), 14 C, 15 the first step's
 // numerator -C - gamma dt dC.v (its square weighs the particle in the residual).
 // Also reduces sum C^2 (the residual's denominator) per block.
-__global__ __launch_bounds__(256) void k_features(KP P, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
+template <bool kPersist>
+__global__ __launch_bounds__(256) void k_features(const KP* __restrict__ kps, WorkTab W, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
                                                   const float4* __restrict__ vin, M3C Fm, M3C Lm,
                                                   const float4* __restrict__ aux, M3P Sp, float4* __restrict__ selfDv,
                                                   const u32* __restrict__ cs, NLc nl,
                                                   const float* __restrict__ lamPrev, float* __restrict__ feat,
                                                   double* __restrict__ part) {
   __shared__ MatLds T;
-  loadMatLds(T, mats);
-  const u32 i = xcdBlock() * 256u + threadIdx.x;
+  u32 curScene = ~0u;
   float c2 = 0.0f;
-  if (i < P.N) {
+  forWork<kPersist>(W, [&](u32 scene, u32 first) {
+    sceneMats(T, mats, scene, &curScene);
+    const KP& P = kps[scene];
+    const u32 i = first + threadIdx.x;
+    if (i >= P.pEnd) return;
     float row[16];
 #pragma unroll
     for (int k = 0; k < 16; k++) row[k] = 0.0f;
@@ -377,7 +418,7 @@ __global__ __launch_bounds__(256) void k_features(KP P, const DevMat* __restrict
     o[1] = make_float4(row[4], row[5], row[6], row[7]);
     o[2] = make_float4(row[8], row[9], row[10], row[11]);
     o[3] = make_float4(row[12], row[13], row[14], row[15]);
-  }
+  });
   blockReduce2(0.0, (double)c2, part + 2 * blockIdx.x);
 }
 
@@ -413,14 +454,15 @@ static inline M3C cm(const M3P& m) { return M3C{m.a, m.b, m.c}; }
 
 // sim.js:255-266 after the reorder; ends with s->vcur on the final velocity
 void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
-  const u32 gN = cdiv(s->N, 256);
-  const KP& P = s->kp;
+  const u32 gN = s->grid;  // blocks of a particle launch (and residual partials)
+  const KP* P = s->dKP;
+  const WorkTab W = s->work();
   PSet& S = s->set[s->cur];
   M3P F{S.Fa, S.Fb, S.Fc};
   hipStream_t st = s->stream;
   int vi = s->vcur;
-  hipLaunchKernelGGL(k_computeL, dim3(gN), dim3(256), 0, st, P, (const float4*)S.pos, (const float4*)s->vel[vi],
-                     (const u32*)s->cellStart, s->L, s->nbrIdx, s->nbrCnt, s->nbrK);
+  XPBI_LAUNCH_W(k_computeL, dim3(gN), dim3(256), 0, st, P, W, (const float4*)S.pos, (const float4*)s->vel[vi],
+                     (const u32*)s->cellStart, s->L, s->nbrIdx, s->nbrCnt, s->nbrK, s->N);
   const NLc nl{s->nbrIdx, s->nbrCnt, s->nbrK, s->N};
   s->prof_mark("computeL");
   if (stop == XPBI_STAGE_COMPUTE_L) return;
@@ -432,7 +474,7 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
   const float* hbHint = s->wsMode == 1 ? (const float*)s->wsHb : nullptr;
   int warm = 0;  // the learned first iteration ran
   if (s->wsMode != 0) {
-    hipLaunchKernelGGL(k_features, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
+    XPBI_LAUNCH_W(k_features, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
                        (const float4*)s->vel[vi], cm(F), cm(s->L), (const float4*)S.aux, s->Sp, s->selfDv,
                        (const u32*)s->cellStart, nl, (const float*)s->lamPrev, s->wsFeat, s->residPart);
     s->prof_mark("features");
@@ -442,16 +484,16 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
       // instead of omega d0: its Sp, selfDv and lambda, then solveB (and PBF)
       launch_mlp_apply(s);
       s->prof_mark("mlp");
-      hipLaunchKernelGGL(k_solveB, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
+      XPBI_LAUNCH_W(k_solveB, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
                          (const float4*)s->vel[vi], cm(s->Sp), (const float4*)s->selfDv, (const u32*)s->cellStart,
                          nl, s->vel[vi ^ 1], (const SolveCtl*)nullptr, 0u);
       s->prof_mark("solveB");
       vi ^= 1;
       if (water) {
-        hipLaunchKernelGGL(k_pbfA, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
+        XPBI_LAUNCH_W(k_pbfA, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
                            (const float4*)s->vel[vi], S.aux, (const u32*)s->cellStart, (const SolveCtl*)nullptr, 0u);
         s->prof_mark("pbfA");
-        hipLaunchKernelGGL(k_pbfB, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
+        XPBI_LAUNCH_W(k_pbfB, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
                            (const float4*)s->vel[vi], (const float4*)S.aux, (const u32*)s->cellStart,
                            s->vel[vi ^ 1], (const SolveCtl*)nullptr, 0u);
         s->prof_mark("pbfB");
@@ -470,7 +512,7 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
     }
     double* out = (s->residOn && k < kMaxResidIters) ? s->residOut + 2 * k : nullptr;
     double* part = (out || loop) ? s->residPart : nullptr;
-    hipLaunchKernelGGL(k_solveA, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
+    XPBI_LAUNCH_W(k_solveA, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
                        (const float4*)s->vel[vi], cm(F), cm(s->L), S.aux, s->Sp, s->selfDv,
                        (const u32*)s->cellStart, nl, part, (const SolveCtl*)ctl, (u32)k, hbHint);
     s->prof_mark("solveA");
@@ -478,16 +520,16 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
       hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(256), 0, st, (const double*)part, gN, out, ctl, (u32)k, tol2);
       if (out) s->residIters = k + 1;
     }
-    hipLaunchKernelGGL(k_solveB, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
+    XPBI_LAUNCH_W(k_solveB, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
                        (const float4*)s->vel[vi], cm(s->Sp), (const float4*)s->selfDv, (const u32*)s->cellStart,
                        nl, s->vel[vi ^ 1], (const SolveCtl*)ctl, (u32)k);
     s->prof_mark("solveB");
     vi ^= 1;
     if (water) {
-      hipLaunchKernelGGL(k_pbfA, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
+      XPBI_LAUNCH_W(k_pbfA, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
                          (const float4*)s->vel[vi], S.aux, (const u32*)s->cellStart, (const SolveCtl*)ctl, (u32)k);
       s->prof_mark("pbfA");
-      hipLaunchKernelGGL(k_pbfB, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
+      XPBI_LAUNCH_W(k_pbfB, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
                          (const float4*)s->vel[vi], (const float4*)S.aux, (const u32*)s->cellStart, s->vel[vi ^ 1],
                          (const SolveCtl*)ctl, (u32)k);
       s->prof_mark("pbfB");
@@ -496,26 +538,26 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
     launched++;
   }
   if (loop && !water && launched > 0) {
-    hipLaunchKernelGGL(k_parity_fix, dim3(gN), dim3(256), 0, st, (const SolveCtl*)ctl, (u32)launched, s->N,
+    hipLaunchKernelGGL(k_parity_fix, dim3(cdiv(s->N, 256)), dim3(256), 0, st, (const SolveCtl*)ctl, (u32)launched, s->N,
                        (const float4*)s->vel[vi ^ 1], s->vel[vi]);
     s->prof_mark("parityFix");
   }
   s->lastIters = launched;
   s->warmIters = warm;
   if (s->wsMode == 2)
-    hipLaunchKernelGGL(k_collect_targets, dim3(gN), dim3(256), 0, st, (const float*)s->wsFeat,
+    hipLaunchKernelGGL(k_collect_targets, dim3(cdiv(s->N, 256)), dim3(256), 0, st, (const float*)s->wsFeat,
                        (const float4*)S.aux, s->N, s->wsTarget);
   if (stop == XPBI_STAGE_SOLVE) { s->vcur = vi; return; }
-  hipLaunchKernelGGL(k_xsph, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
+  XPBI_LAUNCH_W(k_xsph, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
                      (const float4*)s->vel[vi], (const u32*)s->cellStart, nl, s->vel[vi ^ 1]);
   s->prof_mark("xsph");
   vi ^= 1;
   if (stop == XPBI_STAGE_XSPH) { s->vcur = vi; return; }
-  hipLaunchKernelGGL(k_updateF, dim3(gN), dim3(256), 0, st, P, (const DevMat*)s->dMats, (const float4*)S.pos,
+  XPBI_LAUNCH_W(k_updateF, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
                      (const float4*)s->vel[vi], F, cm(s->L), S.aux, (const u32*)s->cellStart, nl, hbHint);
   s->prof_mark("updateF");
   if (stop == XPBI_STAGE_UPDATE_F) { s->vcur = vi; return; }
-  hipLaunchKernelGGL(k_integrate, dim3(gN), dim3(256), 0, st, P, S.pos, s->vel[vi]);
+  XPBI_LAUNCH_W(k_integrate, dim3(gN), dim3(256), 0, st, P, W, S.pos, s->vel[vi]);
   s->prof_mark("integrate");
   s->vcur = vi;
 }

@@ -33,14 +33,18 @@ __global__ __launch_bounds__(256) void k_fill_u32(u32* __restrict__ p, u32 n, u3
 }
 
 // countParticles (simkernels.js:79-84) + key output for the radix sort
-__global__ __launch_bounds__(256) void k_hash_count(KP P, const float4* __restrict__ pos, u32* __restrict__ keys,
+template <bool kPersist>
+__global__ __launch_bounds__(256) void k_hash_count(const KP* __restrict__ kps, WorkTab W, const float4* __restrict__ pos, u32* __restrict__ keys,
                                                     u32* __restrict__ count) {
-  u32 i = blockIdx.x * 256u + threadIdx.x;
-  if (i >= P.N) return;
-  float4 p = pos[i];
-  u32 c = cellIndex(P, cellCoord(P, f3{p.x, p.y, p.z}));
-  keys[i] = c;
-  atomicAdd(&count[c], 1u);
+  forWork<kPersist>(W, [&](u32 scene, u32 first) {
+    const KP& P = kps[scene];
+    const u32 i = first + threadIdx.x;
+    if (i >= P.pEnd) return;
+    float4 p = pos[i];
+    u32 c = cellIndex(P, cellCoord(P, f3{p.x, p.y, p.z}));
+    keys[i] = c;
+    atomicAdd(&count[c], 1u);
+  });
 }
 
 // ---------------------------------------------------------------- scan
@@ -181,30 +185,34 @@ __global__ __launch_bounds__(RADIX_T) void k_radix_scatter(const u32* __restrict
 }
 
 // ---------------------------------------------------------------- reorder
-__global__ __launch_bounds__(256) void k_reorder(KP P, const u32* __restrict__ perm, PSetC src,
+template <bool kPersist>
+__global__ __launch_bounds__(256) void k_reorder(const KP* __restrict__ kps, WorkTab W, const u32* __restrict__ perm, PSetC src,
                                                  const float4* __restrict__ vsrc, PSet dst, float4* __restrict__ vdst,
                                                  float* __restrict__ lamPrev) {
-  u32 j = xcdBlock() * 256u + threadIdx.x;
-  if (j >= P.N) return;
-  u32 i = perm[j];
-  float4 p = src.pos[i];
-  float4 v = vsrc[i];
-  float4 a = src.aux[i];
-  f33 F = ldm(src.Fa, src.Fb, src.Fc, i);
-  u32 id = src.id[i];
-  float Vn = P.V0 * wclamp(det3(F), 0.1f, 10.0f);
-  v.x = v.x + P.dt * P.grx;
-  v.y = v.y + P.dt * P.gry;
-  v.z = v.z + P.dt * P.grz;
-  v.w = Vn;
-  a.x = Vn;
-  if (lamPrev) lamPrev[j] = a.y;  // the previous substep's final lambda, for the warm start
-  a.y = 0.0f;
-  dst.pos[j] = p;
-  vdst[j] = v;
-  dst.aux[j] = a;
-  stm(M3P{dst.Fa, dst.Fb, dst.Fc}, j, F);
-  dst.id[j] = id;
+  forWork<kPersist>(W, [&](u32 scene, u32 first) {
+    const KP& P = kps[scene];
+    const u32 j = first + threadIdx.x;
+    if (j >= P.pEnd) return;
+    u32 i = perm[j];
+    float4 p = src.pos[i];
+    float4 v = vsrc[i];
+    float4 a = src.aux[i];
+    f33 F = ldm(src.Fa, src.Fb, src.Fc, i);
+    u32 id = src.id[i];
+    float Vn = P.V0 * wclamp(det3(F), 0.1f, 10.0f);
+    v.x = v.x + P.dt * P.grx;
+    v.y = v.y + P.dt * P.gry;
+    v.z = v.z + P.dt * P.grz;
+    v.w = Vn;
+    a.x = Vn;
+    if (lamPrev) lamPrev[j] = a.y;  // the previous substep's final lambda, for the warm start
+    a.y = 0.0f;
+    dst.pos[j] = p;
+    vdst[j] = v;
+    dst.aux[j] = a;
+    stm(M3P{dst.Fa, dst.Fb, dst.Fc}, j, F);
+    dst.id[j] = id;
+  });
 }
 
 // plain float4 copy (bandwidth ceiling / counter calibration)
@@ -239,11 +247,10 @@ int radix_passes(u32 C) {
 
 void launch_sort(Sim* s) {
   const u32 N = s->N, C = s->C;
-  const u32 gN = cdiv(N, 256);
   hipLaunchKernelGGL(k_fill_u32, dim3(cdiv(C, 256)), dim3(256), 0, s->stream, s->count, C, 0u);
   s->prof_mark("clear");
-  hipLaunchKernelGGL(k_hash_count, dim3(gN), dim3(256), 0, s->stream, s->kp, (const float4*)s->set[s->cur].pos,
-                     s->keys[0], s->count);
+  XPBI_LAUNCH_W(k_hash_count, dim3(s->grid), dim3(256), 0, s->stream, (const KP*)s->dKP, s->work(),
+                     (const float4*)s->set[s->cur].pos, s->keys[0], s->count);
   s->prof_mark("countParticles");
   launch_exclusive_scan(s, s->count, C, s->cellStart, true, N);
   s->prof_mark("scanCells");
@@ -267,11 +274,11 @@ void launch_sort(Sim* s) {
 }
 
 void launch_reorder(Sim* s) {
-  const u32 gN = cdiv(s->N, 256);
   int src = s->cur, dst = s->cur ^ 1;
   PSet& S = s->set[src];
   PSetC sc{S.pos, S.Fa, S.Fb, S.Fc, S.aux, S.id};
-  hipLaunchKernelGGL(k_reorder, dim3(gN), dim3(256), 0, s->stream, s->kp, (const u32*)s->perm, sc,
+  XPBI_LAUNCH_W(k_reorder, dim3(s->grid), dim3(256), 0, s->stream, (const KP*)s->dKP, s->work(),
+                     (const u32*)s->perm, sc,
                      (const float4*)s->vel[s->vcur], s->set[dst], s->vel[s->vcur ^ 1],
                      s->wsMode != 0 ? s->lamPrev : (float*)nullptr);
   s->cur = dst;

@@ -80,6 +80,17 @@ SIGNATURES = {
     "xpbi_set_residual_monitor": (C.c_int, [_P, C.c_int]),
     "xpbi_read_residuals": (C.c_int, [_P, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_int)]),
     "xpbi_set_tolerance": (C.c_int, [_P, C.c_float, C.c_int]),
+    "xpbi_create_batch": (C.c_int, [C.POINTER(SceneDesc), C.c_int, C.POINTER(_P)]),
+    "xpbi_scene_count": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "xpbi_scene_info": (C.c_int, [_P, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                  C.POINTER(C.c_uint32)]),
+    "xpbi_write_scene_materials": (C.c_int, [_P, C.c_int, C.POINTER(Material), C.c_int]),
+    "xpbi_write_scene_params": (C.c_int, [_P, C.c_int, C.POINTER(Params)]),
+    "xpbi_set_launch": (C.c_int, [_P, C.c_int]),
+    "xpbi_get_launch": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "xpbi_graph_step": (C.c_int, [_P, C.c_int, C.c_int, C.c_int]),
+    "xpbi_autotune": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
     "xpbi_last_iterations": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "xpbi_set_warmstart": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t]),
     "xpbi_collect_warmstart_pairs": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float),

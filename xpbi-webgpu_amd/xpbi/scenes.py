@@ -313,3 +313,48 @@ def config_c5(seed=5, dt=1.0 / 6000.0, iterations=20):
     s["collider"] = dict(pos=[0.5, 0.62, 0.5], R=0.05, vel=[0.0, -1.0, 0.0])
     return _finish(s, out, [PLASTICINE(5e3, E=2e5, nu=0.3, density=2000.0)], seed, False,
                    info="von Mises block 100^3 + poke")
+
+
+def config_c4(seed=4, count=64, dt=1.0 / 6000.0, iterations=20, r=0.01):
+    """Batched ensemble: `count` independent scenes with particle counts log-uniform in
+    2k-60k (seeded), each a jittered block dropped on its own floor, its material drawn
+    uniformly from Drucker-Prager (phi 20-40 deg), NACC (xi 0-10, beta 0.3), von Mises
+    (sigmaY 1e3-1e4 Pa), Herschel-Bulkley (h 0.3-1.0, sigmaY 50-500 Pa, eta 10-100) and
+    the snow clamp (theta_c 2.5e-2, theta_s 7.5e-3, xi 10); E 2e5 Pa, nu 0.3, rho 2000.
+    Each block stands on its floor, so it compresses and flows from the first substeps.
+    Returns the list of scene dicts."""
+    pick = Rng(seed, 1000)
+    out_scenes = []
+    for q in range(count):
+        u = pick.take(4)
+        n = int(round(math.exp(math.log(2000) + u[0] * (math.log(60000) - math.log(2000)))))
+        kind = int(u[1] * 5)
+        v = u[2]
+        if kind == 0:
+            m, st = SAND(20 + 20 * v, 0.0, E=2e5, nu=0.3, density=2000.0), 0.0
+        elif kind == 1:
+            alpha0 = -0.01
+            m = dict(kind=MAT["NACC"], E=2e5, nu=0.3, density=2000.0, p0=alpha0, p1=0.3, p2=10.0 * v, p3=1.85,
+                     color=[0.85, 0.9, 1.0])
+            st = alpha0
+        elif kind == 2:
+            m, st = PLASTICINE(1e3 * 10 ** v, E=2e5, nu=0.3, density=2000.0), 0.0
+        elif kind == 3:
+            m = GOO(0.3 + 0.7 * v, 50 + 450 * u[3], 10 + 90 * v, E=2e5, nu=0.3, density=2000.0)
+            st = 0.0
+        else:
+            m, st = SNOW_CLASSIC(2.5e-2, 7.5e-3, 10.0, E=2e5, nu=0.3, density=2000.0), 1.0
+        # a block of about n particles, twice as tall as wide
+        nx = max(4, int(round((n / 2.0) ** (1.0 / 3.0))))
+        ny = max(4, int(round(n / (nx * nx))))
+        ext = [nx * r, ny * r, nx * r]
+        dom = [ext[0] * 3.0, ext[1] * 1.3 + 4 * r, ext[2] * 3.0]
+        s = _solver(r, dt, iterations)
+        s["domainMin"], s["domainMax"] = [0.0, 0.0, 0.0], dom
+        mn = [ext[0], 0.0, ext[2]]
+        mx = [mn[0] + ext[0], mn[1] + ext[1], mn[2] + ext[2]]
+        o, rng = _new(), Rng(seed, q)
+        sample_box(o, mn, mx, r, 0, rng, 0.1, state0=st)
+        out_scenes.append(_finish(s, o, [m], seed * 1000 + q, False,
+                                  info=f"ensemble scene {q}: {m['kind']} x {nx}x{ny}x{nx}"))
+    return out_scenes

@@ -37,91 +37,121 @@ def pack_materials(materials, V0):
     return arr
 
 
-class Simulation:
-    """One scene resident on one GPU (the reference's ``new Simulation(device, scene)``)."""
+def _scene_desc(scene, device, keep):
+    """SceneDesc of one scene dict; arrays it points to are appended to `keep`."""
+    pos = np.ascontiguousarray(scene["positions"], dtype=np.float32).reshape(-1, 3)
+    d = _abi.SceneDesc()
+    d.N = pos.shape[0]
+    d.positions = _fptr(pos)
+    keep.append(pos)
+    tags = scene.get("tags")
+    if tags is not None:
+        tags = np.ascontiguousarray(tags, dtype=np.float32)
+        d.tags = _fptr(tags)
+        keep.append(tags)
+    mids = np.ascontiguousarray(scene.get("matIds", np.zeros(d.N)), dtype=np.uint32)
+    d.matIds = mids.ctypes.data_as(C.POINTER(C.c_uint32))
+    keep.append(mids)
+    if scene.get("states") is not None:
+        st = np.ascontiguousarray(scene["states"], dtype=np.float32)
+        d.states = _fptr(st)
+        keep.append(st)
+    if scene.get("velocities") is not None:
+        v = np.ascontiguousarray(scene["velocities"], dtype=np.float32).reshape(-1, 3)
+        d.velocities = _fptr(v)
+        keep.append(v)
+    d.h = float(scene["h"])
+    for k in range(3):
+        d.domainMin[k] = float(scene["domainMin"][k])
+        d.domainMax[k] = float(scene["domainMax"][k])
+    d.device = device
+    return d
 
-    def __init__(self, scene, device=0):
-        self.scene = scene
-        pos = np.ascontiguousarray(scene["positions"], dtype=np.float32).reshape(-1, 3)
-        self.N = pos.shape[0]
-        self.h = float(scene["h"])
-        self.support = 2 * self.h
-        self.V0 = self.h * self.h * self.h
-        self._keep = []
-        d = _abi.SceneDesc()
-        d.N = self.N
-        d.positions = _fptr(pos)
-        self._keep.append(pos)
-        tags = scene.get("tags")
-        if tags is not None:
-            tags = np.ascontiguousarray(tags, dtype=np.float32)
-            d.tags = _fptr(tags)
-            self._keep.append(tags)
-        mids = np.ascontiguousarray(scene.get("matIds", np.zeros(self.N)), dtype=np.uint32)
-        d.matIds = mids.ctypes.data_as(C.POINTER(C.c_uint32))
-        self._keep.append(mids)
-        if scene.get("states") is not None:
-            st = np.ascontiguousarray(scene["states"], dtype=np.float32)
-            d.states = _fptr(st)
-            self._keep.append(st)
-        if scene.get("velocities") is not None:
-            v = np.ascontiguousarray(scene["velocities"], dtype=np.float32).reshape(-1, 3)
-            d.velocities = _fptr(v)
-            self._keep.append(v)
-        d.h = self.h
-        for k in range(3):
-            d.domainMin[k] = float(scene["domainMin"][k])
-            d.domainMax[k] = float(scene["domainMax"][k])
-        d.device = device
-        h = C.c_void_p()
-        check(lib().xpbi_create(C.byref(d), C.byref(h)), None, "xpbi_create")
-        self._h = h
-        self._keep = None
+
+class SceneInfo:
+    """Grid and particle range of one scene of a simulation."""
+
+    def __init__(self, sim_handle, q, scene):
         o = (C.c_float * 3)()
         cs = C.c_float()
         dims = (C.c_int32 * 3)()
-        nc = C.c_uint32()
-        n = C.c_uint32()
-        check(lib().xpbi_grid_info(self._h, o, C.byref(cs), dims, C.byref(nc), C.byref(n)), self._h)
-        self.gridOrigin = list(o)
-        self.cellSize = cs.value
-        self.gridDims = list(dims)
-        self.numCells = nc.value
-        self.writeMaterials(scene["materials"])
+        nc, n, fp, fc = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(lib().xpbi_scene_info(sim_handle, q, o, C.byref(cs), dims, C.byref(nc), C.byref(n), C.byref(fp),
+                                    C.byref(fc)), sim_handle)
+        self.scene = scene
+        self.gridOrigin, self.cellSize, self.gridDims = list(o), cs.value, list(dims)
+        self.numCells, self.N, self.first, self.firstCell = nc.value, n.value, fp.value, fc.value
+        self.h = float(scene["h"])
+        self.support = 2 * self.h
+        self.V0 = self.h * self.h * self.h
+
+    @property
+    def rows(self):
+        return slice(self.first, self.first + self.N)
+
+
+class Simulation:
+    """One scene resident on one GPU (the reference's ``new Simulation(device, scene)``), or,
+    given a list of scenes, a batch: independent scenes concatenated in one simulation whose
+    every kernel is one launch for all of them (fields are read and written whole, scene
+    after scene; ``scenes_info[q].rows`` is scene q's slice)."""
+
+    def __init__(self, scene, device=0):
+        scenes = list(scene) if isinstance(scene, (list, tuple)) else [scene]
+        self.scene = scenes[0]
+        keep = []
+        descs = (_abi.SceneDesc * len(scenes))()
+        for q, sc in enumerate(scenes):
+            descs[q] = _scene_desc(sc, device, keep)
+        h = C.c_void_p()
+        check(lib().xpbi_create_batch(descs, len(scenes), C.byref(h)), None, "xpbi_create_batch")
+        self._h = h
+        self.scenes_info = [SceneInfo(h, q, sc) for q, sc in enumerate(scenes)]
+        self.N = sum(si.N for si in self.scenes_info)
+        s0 = self.scenes_info[0]
+        self.h, self.support, self.V0 = s0.h, s0.support, s0.V0
+        self.gridOrigin, self.cellSize, self.gridDims, self.numCells = (s0.gridOrigin, s0.cellSize, s0.gridDims,
+                                                                        s0.numCells)
+        self.totalCells = sum(si.numCells for si in self.scenes_info)
+        for q, sc in enumerate(scenes):
+            self.writeMaterials(sc["materials"], scene=q)
 
     # -- reference surface -------------------------------------------------
-    def writeMaterials(self, materials):
-        arr = pack_materials(materials, self.V0)
-        check(lib().xpbi_write_materials(self._h, arr, len(materials)), self._h, "writeMaterials")
+    def writeMaterials(self, materials, scene=0):
+        arr = pack_materials(materials, self.scenes_info[scene].V0)
+        check(lib().xpbi_write_scene_materials(self._h, scene, arr, len(materials)), self._h, "writeMaterials")
 
-    def writeParams(self, p):
+    def writeParams(self, p, scene=0):
         """sim.js:159-186.  p: dt, domainMin/Max, gravity, collider*, xsphC, omega, contactOmega,
         pbfEps, sCorrK, time, kDamp (default 0.4)."""
+        si = self.scenes_info[scene]
         r = _abi.Params()
         for k in range(3):
-            r.gridOrigin[k] = self.gridOrigin[k]
-            r.gridDims[k] = self.gridDims[k]
+            r.gridOrigin[k] = si.gridOrigin[k]
+            r.gridDims[k] = si.gridDims[k]
             r.domainMin[k] = p["domainMin"][k]
             r.domainMax[k] = p["domainMax"][k]
             r.gravity[k] = p["gravity"][k]
             r.colliderPos[k] = p.get("colliderPos", [0.0, 0.0, 0.0])[k]
             r.colliderVel[k] = p.get("colliderVel", [0.0, 0.0, 0.0])[k]
-        r.cellSize = self.cellSize
-        r.numCells = self.numCells
+        r.cellSize = si.cellSize
+        r.numCells = si.numCells
         r.dt = p["dt"]
-        r.h = self.h
-        r.support = self.support
+        r.h = si.h
+        r.support = si.support
         r.colliderR = p.get("colliderR", 0.0)
         r.colliderActive = 1.0 if p.get("colliderActive", False) else 0.0
-        r.N = self.N
-        r.V0 = self.V0
+        r.N = si.N
+        r.V0 = si.V0
         r.xsphC, r.omega = p["xsphC"], p["omega"]
         r.pbfEps, r.sCorrK = p["pbfEps"], p["sCorrK"]
         r.time = p.get("time", 0.0)
         r.contactOmega = p["contactOmega"]
         r.kDamp = p.get("kDamp", 0.4)
-        self.params = r
-        check(lib().xpbi_write_params(self._h, C.byref(r)), self._h, "writeParams")
+        if scene == 0:
+            self.params = r
+        si.params = r
+        check(lib().xpbi_write_scene_params(self._h, scene, C.byref(r)), self._h, "writeParams")
 
     def encodeSubstep(self, iterations, hasWater):
         check(lib().xpbi_substep(self._h, int(iterations), int(bool(hasWater))), self._h, "substep")
@@ -181,7 +211,7 @@ class Simulation:
 
     def read(self, field):
         if field == "CELLSTART":
-            out = np.empty(self.numCells, dtype=np.uint32)
+            out = np.empty(self.totalCells, dtype=np.uint32)
         else:
             w, dt = self._SHAPES[field]
             out = np.empty((self.N, w) if w > 1 else self.N, dtype=dt)
@@ -219,6 +249,27 @@ class Simulation:
         n, r = C.c_int(), C.c_double()
         check(lib().xpbi_last_iterations(self._h, C.byref(n), C.byref(r)), self._h)
         return n.value, r.value
+
+    def set_launch(self, blocks):
+        """0: one block per 256-particle run; > 0: persistent grid of that many blocks."""
+        check(lib().xpbi_set_launch(self._h, int(blocks)), self._h, "set_launch")
+
+    def get_launch(self):
+        b, n = C.c_int(), C.c_int()
+        check(lib().xpbi_get_launch(self._h, C.byref(b), C.byref(n)), self._h)
+        return b.value, n.value
+
+    def graph_step(self, substeps, iterations, hasWater):
+        """step() replayed from a captured HIP graph of two substeps."""
+        check(lib().xpbi_graph_step(self._h, int(substeps), int(iterations), int(bool(hasWater))), self._h,
+              "graph_step")
+
+    def autotune(self, iterations, hasWater, substeps=4):
+        """Pick the fastest launch shape on this simulation; returns (blocks, ms per substep)."""
+        b, ms = C.c_int(), C.c_float()
+        check(lib().xpbi_autotune(self._h, int(iterations), int(bool(hasWater)), int(substeps), C.byref(b),
+                                  C.byref(ms)), self._h, "autotune")
+        return b.value, ms.value
 
     def set_warmstart(self, params):
         """Learned warm start with the given network (canonical float vector of

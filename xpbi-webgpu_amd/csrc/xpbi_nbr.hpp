@@ -29,6 +29,25 @@ __device__ __forceinline__ void forRows(const KP& P, const u32* __restrict__ cs,
   }
 }
 
+// the same 9 spans as forRows, handed over whole: fn(beg, end)
+template <class Fn>
+__device__ __forceinline__ void forRowSpans(const KP& P, const u32* __restrict__ cs, i3 cc, Fn&& fn) {
+  const u32 x0 = (u32)max(cc.x - 1, 0), x1 = (u32)min(cc.x + 1, P.gx - 1);
+#pragma unroll 1
+  for (int dz = -1; dz <= 1; dz++) {
+    int z = cc.z + dz;
+    if (z < 0 || z >= P.gz) continue;
+#pragma unroll 1
+    for (int dy = -1; dy <= 1; dy++) {
+      int y = cc.y + dy;
+      if (y < 0 || y >= P.gy) continue;
+      u32 row = P.cellBase + (u32)y * (u32)P.gx + (u32)z * P.gxy;
+      u32 beg = cs[row + x0];
+      fn(beg, min(cs[row + x1 + 1u], beg + kRowCap));
+    }
+  }
+}
+
 struct NLc {
   const u32* __restrict__ idx;  // idx[k * N + i]
   const u32* __restrict__ cnt;
@@ -65,37 +84,42 @@ __device__ __forceinline__ void forSupport(const KP& P, const u32* __restrict__ 
 #ifndef XPBI_GATHER_BATCH
 #define XPBI_GATHER_BATCH 8
 #endif
+// the constraint gather of solveB carries 17 floats per neighbour: its own depth
+#ifndef XPBI_GATHER_BATCH_B
+#define XPBI_GATHER_BATCH_B 6
+#endif
 constexpr int kBatch = XPBI_GATHER_BATCH;
+constexpr int kBatchB = XPBI_GATHER_BATCH_B;
 
 // Support-neighbour walk with the gathers issued in batches of kBatch
 // independent loads (and the next batch's indices prefetched) so that several
 // cache misses are in flight per lane.  load(b) gathers the neighbour's record,
 // whose member p is (x_b, tag_b); use(b, rec, dxv = x_i - x_b, |dxv|^2)
 // consumes it, in list order.
-template <class Load, class Use>
+template <int kB = kBatch, class Load, class Use>
 __device__ __forceinline__ void forSupportG(const KP& P, const u32* __restrict__ cs, const NLc& nl, u32 i, f3 xi,
                                             Load&& load, Use&& use) {
   const u32 n = nl.cnt[i];
   if (n <= nl.K) {
     const u32* __restrict__ p = nl.idx + i;
-    u32 bn[kBatch];
+    u32 bn[kB];
 #pragma unroll
-    for (int j = 0; j < kBatch; j++) bn[j] = ((u32)j < n) ? p[(size_t)j * nl.N] : i;
-    for (u32 k = 0; k < n; k += kBatch) {
-      u32 b[kBatch];
-      decltype(load(0u)) d[kBatch];
+    for (int j = 0; j < kB; j++) bn[j] = ((u32)j < n) ? p[(size_t)j * nl.N] : i;
+    for (u32 k = 0; k < n; k += kB) {
+      u32 b[kB];
+      decltype(load(0u)) d[kB];
 #pragma unroll
-      for (int j = 0; j < kBatch; j++) {
+      for (int j = 0; j < kB; j++) {
         b[j] = bn[j];
         d[j] = load(b[j]);
       }
 #pragma unroll
-      for (int j = 0; j < kBatch; j++) {
-        u32 kk = k + kBatch + (u32)j;
+      for (int j = 0; j < kB; j++) {
+        u32 kk = k + kB + (u32)j;
         bn[j] = (kk < n) ? p[(size_t)kk * nl.N] : i;
       }
 #pragma unroll
-      for (int j = 0; j < kBatch; j++) {
+      for (int j = 0; j < kB; j++) {
         if (k + (u32)j < n) {
           const f3 dd = xi - xyz(d[j].p);
           use(b[j], d[j], dd, dot(dd, dd));
@@ -376,7 +400,7 @@ __device__ __forceinline__ void solveBGatherGlobal(const KP& P, const MatLds& T,
   const f3 xi = xyz(pi);
   const f3 xci = xi + P.dt * xyz(vi4);
   const float mass_i = T.mass[matOfTag(pi.w)];
-  forSupportG(P, cs, nl, i, xi, [&](u32 b) { return SpGather{pos[b], vin[b], Sp.a[b], Sp.b[b], Sp.c[b]}; },
+  forSupportG<kBatchB>(P, cs, nl, i, xi, [&](u32 b) { return SpGather{pos[b], vin[b], Sp.a[b], Sp.b[b], Sp.c[b]}; },
               [&](u32 b, const SpGather& g, f3 dxv, float r2) {
                 const f33 Spb{{f3{g.a.x, g.a.y, g.a.z}, f3{g.a.w, g.b.x, g.b.y}, f3{g.b.z, g.b.w, g.c}}};
                 solveBPair(P, T, Vi, mass_i, xci, dxv, r2, g.p, g.v, Spb, dv, corr);

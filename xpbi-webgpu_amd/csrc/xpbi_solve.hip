@@ -29,16 +29,35 @@ __global__ __launch_bounds__(256) void k_computeL(const KP* __restrict__ kps, Wo
     f3 xi = xyz(pos[i]);
     f33 A = mzero<float>();
     u32 n = 0;
-    forRows(P, cs, cellCoord(P, xi), [&](u32 b) {
-      if (b != i) {
-        f3 dxv = xi - xyz(pos[b]);
-        float r2 = dot(dxv, dxv);
-        if (r2 < P.supp2) {
-          f3 g = gradWfast(P, dxv, r2);
-          f3 r = -dxv;
-          A = madd(A, mscale(outer(g, r), vin[b].w));
-          if (n < K) nIdx[(size_t)n * NT + i] = b;
-          n++;
+    // candidates in batches of 8: their positions, then the volumes of the
+    // accepted ones, each batch's loads issued together; accumulation and the
+    // list keep the candidate order
+    forRowSpans(P, cs, cellCoord(P, xi), [&](u32 beg, u32 end) {
+      for (u32 b0 = beg; b0 < end; b0 += 8u) {
+        float4 pb[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) pb[j] = (b0 + j < end) ? pos[b0 + j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        f3 d[8];
+        float r2[8];
+        bool ok[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          d[j] = xi - xyz(pb[j]);
+          r2[j] = dot(d[j], d[j]);
+          ok[j] = (b0 + j < end) && (b0 + j != i) && (r2[j] < P.supp2);
+        }
+        float Vb[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) Vb[j] = ok[j] ? vin[b0 + j].w : 0.0f;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          if (ok[j]) {
+            f3 g = gradWfast(P, d[j], r2[j]);
+            f3 r = -d[j];
+            A = madd(A, mscale(outer(g, r), Vb[j]));
+            if (n < K) nIdx[(size_t)n * NT + i] = b0 + j;
+            n++;
+          }
         }
       }
     });

@@ -37,7 +37,7 @@ ALG_BYTES = {"solveA": 164, "solveB": 96, "computeL": 68, "xsph": 48, "updateF":
 GATHER = ("computeL", "solveA", "solveB", "xsph", "updateF")
 
 
-def to_residual(sim, snap, params, tol, cap, substeps):
+def to_residual(sim, snap, params, tol, cap, substeps, before=None):
     """From the snapshot, `substeps` substeps with Jacobi run to residual `tol`
     (cap `cap`), plain (params None) or with the learned warm start.  Returns the
     iterations of each substep, wall ms per substep (synchronised per substep to
@@ -50,7 +50,9 @@ def to_residual(sim, snap, params, tol, cap, substeps):
     its = []
     sim.sync()
     t0 = time.perf_counter()
-    for _ in range(substeps):
+    for k in range(substeps):
+        if before:
+            before(k)
         sim.substep(cap, False)
         its.append(sim.last_iterations()[0])
     sim.sync()
@@ -63,24 +65,26 @@ def to_residual(sim, snap, params, tol, cap, substeps):
     return its, ms, out
 
 
-def learned_vs_plain(sim, tol, cap, substeps, weights=None, log=None):
-    import numpy as np
+def trained_network(weights=None, log=None):
+    """The warm-start network: trained here on the GPU (pairs of the training columns),
+    or loaded from `weights`.  Returns (params, training record)."""
     from xpbi import warmstart
     t0 = time.perf_counter()
-    train = None
     if weights:
-        params = warmstart.load(weights)
-        train = dict(weights=os.path.basename(weights))
-    else:
-        X, T, W = warmstart.training_set(log=log)
-        t1 = time.perf_counter()
-        params, losses = warmstart.train(X, T, W, epochs=30)
-        train = dict(scenes=[c[0] for c in warmstart.training_columns()], pairs=int(len(X)), epochs=30,
-                     loss_first_epoch=float(losses[0]), loss_last_epoch=float(losses[-1]),
-                     collect_s=round(t1 - t0, 2), train_s=round(time.perf_counter() - t1, 2))
+        return warmstart.load(weights), dict(weights=os.path.basename(weights))
+    X, T, W = warmstart.training_set(log=log)
+    t1 = time.perf_counter()
+    params, losses = warmstart.train(X, T, W, epochs=30)
+    return params, dict(scenes=[c[0] for c in warmstart.training_columns()], pairs=int(len(X)), epochs=30,
+                        loss_first_epoch=float(losses[0]), loss_last_epoch=float(losses[-1]),
+                        collect_s=round(t1 - t0, 2), train_s=round(time.perf_counter() - t1, 2))
+
+
+def learned_vs_plain(sim, tol, cap, substeps, params, train, before=None):
+    import numpy as np
     snap = {k: sim.read(k) for k in ("POS", "VEL", "F", "AUX", "ID")}
-    pit, pms, ppos = to_residual(sim, snap, None, tol, cap, substeps)
-    lit, lms, lpos = to_residual(sim, snap, params, tol, cap, substeps)
+    pit, pms, ppos = to_residual(sim, snap, None, tol, cap, substeps, before)
+    lit, lms, lpos = to_residual(sim, snap, params, tol, cap, substeps, before)
     pi, li = float(np.mean(pit)), float(np.mean(lit))
     return dict(tol=tol, cap=cap, substeps=substeps, plain_iterations=pi, learned_iterations=li,
                 iteration_ratio=round(pi / li, 3), plain_ms_per_substep=round(pms, 4),
@@ -108,6 +112,70 @@ def cpu_baseline(sim, scene, iterations, threads):
     return dict(value=sim.N / dt, unit="particle-substeps/s", cores=threads, kind="port",
                 sample=f"1 substep ({iterations} Jacobi iterations) of the settled {sim.N}-particle column, "
                        f"f32 C++ oracle, OpenMP {threads} threads, {dt:.1f} s")
+
+
+def mixed_scene(settle, steps):
+    """Config 3: dam breach, 600k PBF water beside a 400k DP sand wall (phi 35), 10 iterations."""
+    import xpbi
+    from xpbi import scenes
+    sc = scenes.config_c3()
+    I = sc["iterations"]
+    sim = xpbi.Simulation(sc, device=0)
+    sim.writeParams(xpbi.params_from_scene(sc))
+    sim.step(settle, I, True)
+    sim.graph_step(2, I, True)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.graph_step(steps, I, True)
+    sim.sync()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    total, per = sim.time_substeps_detail(4, I, True)
+    N = sim.N
+    kern = {k: round(v[0] / max(v[1], 1) * 1e3, 1) for k, v in per.items()}
+    g = ("computeL", "solveA", "solveB", "pbfA", "pbfB", "xsph", "updateF")
+    g_ms = sum(per[k][0] for k in g if k in per) / 4
+    g_bytes = (264 + 348 * I) * N  # SURVEY upper bound of the gather kernels with water
+    out = dict(particles=N, water=int((sc["matIds"] == 1).sum()), sand=int((sc["matIds"] == 0).sum()),
+               iterations=I, settle_substeps=settle, ms_per_substep_graph=round(ms, 4),
+               particle_substeps_per_s=round(N / (ms / 1e3), 1), kernels_us=kern,
+               gather_kernels_GBs=round(g_bytes / (g_ms / 1e3) / 1e9, 1),
+               gather_kernels_frac=round(g_bytes / (g_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4))
+    sim.close()
+    return out
+
+
+def poke_scene(params, train, settle, substeps, cap):
+    """Config 5: von Mises block 100^3 (sigmaY 5e3, E 2e5) poked by a 0.05 m sphere moving
+    down at 1 m/s (its position updated every substep): learned vs plain Jacobi to 1e-4."""
+    import xpbi
+    from xpbi import scenes
+    sc = scenes.config_c5()
+    sim = xpbi.Simulation(sc, device=0)
+    col = sc["collider"]
+    dt = sc["dt"]
+    state = dict(t=0.0)
+
+    def place(k, t0=None):
+        t = state["t"] if t0 is None else t0
+        pos = [col["pos"][j] + col["vel"][j] * t for j in range(3)]
+        sim.writeParams(xpbi.params_from_scene(sc, collider=dict(pos=pos, R=col["R"], vel=col["vel"])))
+
+    for k in range(settle):
+        place(k)
+        sim.substep(sc["iterations"], False)
+        state["t"] += dt
+    t_start = state["t"]
+
+    def before(k):
+        place(k, t_start + k * dt)
+
+    out = learned_vs_plain(sim, 1e-4, cap, substeps, params, train, before)
+    out.update(particles=sim.N, settle_substeps=settle, collider=dict(R=col["R"], speed_m_s=1.0),
+               note="collider moved every substep; learned iterations count the learned first iteration; "
+                    "network trained on the DP sand columns only")
+    out.pop("training", None)
+    sim.close()
+    return out
 
 
 def ensemble(I, settle, steps):
@@ -155,6 +223,7 @@ def main():
     ap.add_argument("--no-residual", action="store_true")
     ap.add_argument("--config", default="headline", choices=["headline", "c2"])
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the mixed (c3) and poke (c5) scenes")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the substep engine has no multi-GPU path in this build "
@@ -208,11 +277,11 @@ def main():
     gather = dict(achieved_GBs=round(g_bytes / (g_ms / 1e3) / 1e9, 1),
                   frac=round(g_bytes / (g_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4))
 
-    resid = None
+    resid, net, train = None, None, None
     if not args.no_residual:
         print("[bench] Jacobi to residual 1e-4: plain vs learned warm start", file=sys.stderr, flush=True)
-        resid = learned_vs_plain(sim, 1e-4, args.residual_cap, args.residual_substeps, args.weights,
-                                 log=lambda m: print("[bench] " + m, file=sys.stderr, flush=True))
+        net, train = trained_network(args.weights, log=lambda m: print("[bench] " + m, file=sys.stderr, flush=True))
+        resid = learned_vs_plain(sim, 1e-4, args.residual_cap, args.residual_substeps, net, train)
 
     copy_bytes = 1 << 30
     copy_ms = sim.copy_bandwidth(copy_bytes, reps=5)
@@ -229,6 +298,13 @@ def main():
     if not args.no_ensemble:
         print("[bench] 64-scene batched ensemble", file=sys.stderr, flush=True)
         ens = ensemble(I, 60, 20)
+    mixed = poke = None
+    if not args.no_configs:
+        print("[bench] mixed water + sand scene", file=sys.stderr, flush=True)
+        mixed = mixed_scene(60, 10)
+        if net is not None:
+            print("[bench] von Mises block + poke: plain vs learned", file=sys.stderr, flush=True)
+            poke = poke_scene(net, train, 200, 20, args.residual_cap)
 
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "particle-substeps/s", "n_gpus": 1,
@@ -243,7 +319,8 @@ def main():
         "eager_ms_per_step": round(total_ms / detail_steps, 4),
         "roofline": roofline, "gather_kernels_roofline": gather, "kernels": kern,
         "stream_copy_GBs": round(copy_GBs, 1),
-        "jacobi_to_residual": resid, "batched_ensemble": ens, "cpu_baseline": cpu,
+        "jacobi_to_residual": resid, "batched_ensemble": ens, "mixed_water_sand": mixed,
+        "vm_block_poke_to_residual": poke, "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 

@@ -260,7 +260,8 @@ int xpbi_last_iterations(xpbi_sim* sim, int* iterations, double* residual);
  * b3[2], row = output unit) maps each solid particle's features at lambda = 0 to
  * (rho, t): the warm start replaces the first Jacobi iteration's step
  * Delta-lambda = omega d0 (d0 = (-C - gamma dt dC.v) / ((1 + gamma) denom + alpha~)
- * at lambda = 0) by lambda0 = clamp(rho, +-8) d0 and applies it like any
+ * at lambda = 0) by lambda0 = clamp(rho, 0, 2) d0 (at most twice the Newton step,
+ * whatever the network outputs) and applies it like any
  * iteration (solveB, and PBF with water); plain Jacobi then continues, to the
  * tolerance of xpbi_set_tolerance measured against the same ||C|| at
  * lambda = 0 as a plain run, or for the remaining iterations of a fixed count

@@ -1098,7 +1098,7 @@ template <class T> struct Sim {
       c0 = features();
       c0preset = true;
       if (wsMode == 1 && iterations > 0) {
-        // the first Jacobi iteration with Delta-lambda = lambda0 = clamp(y0, +-8) d0
+        // the first Jacobi iteration with Delta-lambda = lambda0 = clamp(y0, 0, 2) d0
         // in place of omega d0, then plain Jacobi
         std::vector<T> lam0(P.N);
         hbHint.assign(P.N, T(-1));
@@ -1106,7 +1106,7 @@ template <class T> struct Sim {
           const T* row = feat + 16 * (size_t)i;
           double y[2];
           mlpForward(wsParams, row, y);
-          lam0[i] = row[12] == T(0) ? T(0) : wclamp(T(float(y[0])), T(-8), T(8)) * row[12];
+          lam0[i] = row[12] == T(0) ? T(0) : wclamp(T(float(y[0])), T(0), T(2)) * row[12];
           hbHint[i] = T(float(y[1]));
         }
         solveA(-1, lam0.data());

@@ -35,3 +35,12 @@ def test_js_frames_match_python(tmp_path, name):
     assert np.array_equal(js_id, sim.read("ID"))
     assert np.array_equal(js_pos, sim.read("POS"))
     sim.close()
+    # the Python frame driver, its chunks replayed from a HIP graph, lands on the same state
+    from xpbi.frame import FrameDriver
+    sim = xpbi.Simulation(scene, device=0)
+    fd = FrameDriver(sim, scene, graph=True)
+    for _ in range(frames):
+        fd.frame()
+    assert np.array_equal(js_id, sim.read("ID"))
+    assert np.array_equal(js_pos, sim.read("POS"))
+    sim.close()

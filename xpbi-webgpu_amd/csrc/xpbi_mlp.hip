@@ -264,7 +264,7 @@ __global__ __launch_bounds__(256) void k_mlp_pack(const float* __restrict__ w, M
 }
 
 // ---------------------------------------------------------------- warm start in the substep
-// lambda0 = clamp(y0, -8, 8) d0 per particle (d0 = feature row slot 12, zero for
+// lambda0 = clamp(y0, 0, 2) d0 per particle (d0 = feature row slot 12, zero for
 // particles without an active constraint); the unscaled Sp / selfDv written by
 // the features kernel are scaled by lambda0, and y1 is the HB bracket position.
 __global__ __launch_bounds__(kMlpBlock) void k_mlp_apply(const MlpPacked* __restrict__ Wg, const float* __restrict__ X,
@@ -280,7 +280,7 @@ __global__ __launch_bounds__(kMlpBlock) void k_mlp_apply(const MlpPacked* __rest
   mlp_forward(W, x8, a);
   if (l < 32u && n < N) {
     const float d0 = X[(u64)n * kFeatStride + 12];
-    const float lam = d0 == 0.0f ? 0.0f : fminf(fmaxf(a.y[0], -8.0f), 8.0f) * d0;
+    const float lam = d0 == 0.0f ? 0.0f : fminf(fmaxf(a.y[0], 0.0f), 2.0f) * d0;
     aux[n].y = lam;
     float4 A = Sp.a[n], B = Sp.b[n];
     Sp.a[n] = make_float4(A.x * lam, A.y * lam, A.z * lam, A.w * lam);

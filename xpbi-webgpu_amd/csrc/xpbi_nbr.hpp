@@ -48,6 +48,23 @@ __device__ __forceinline__ void forRowSpans(const KP& P, const u32* __restrict__
   }
 }
 
+// forRows with the candidates' records loaded kB at a time (all loads of a batch
+// in flight together); use(b, record) in candidate order
+template <int kB, class Load, class Use>
+__device__ __forceinline__ void forRowsG(const KP& P, const u32* __restrict__ cs, i3 cc, Load&& load, Use&& use) {
+  forRowSpans(P, cs, cc, [&](u32 beg, u32 end) {
+    for (u32 b0 = beg; b0 < end; b0 += (u32)kB) {
+      decltype(load(0u)) d[kB];
+#pragma unroll
+      for (int j = 0; j < kB; j++)
+        if (b0 + j < end) d[j] = load(b0 + j);
+#pragma unroll
+      for (int j = 0; j < kB; j++)
+        if (b0 + j < end) use(b0 + j, d[j]);
+    }
+  });
+}
+
 struct NLc {
   const u32* __restrict__ idx;  // idx[k * N + i]
   const u32* __restrict__ cnt;

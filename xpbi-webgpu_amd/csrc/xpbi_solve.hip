@@ -176,6 +176,11 @@ __global__ __launch_bounds__(256) void k_solveB(const KP* __restrict__ kps, Work
 }
 
 // ---------------------------------------------------------------- PBF
+// a candidate of pbfB: position/tag, velocity, PBF multiplier
+struct PVL {
+  float4 p, v;
+  float l;
+};
 // simkernels.js:407-439
 template <bool kPersist>
 __global__ __launch_bounds__(256) void k_pbfA(const KP* __restrict__ kps, WorkTab W, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
@@ -198,10 +203,10 @@ __global__ __launch_bounds__(256) void k_pbfA(const KP* __restrict__ kps, WorkTa
     float rho = T.mass[mi] * P.poly6_0;
     f3 gradSum = splat3(0.0f);
     float grad2 = 0.0f;
-    forRows(P, cs, cellCoord(P, xn), [&](u32 b) {
+    forRowsG<8>(P, cs, cellCoord(P, xn), [&](u32 b) { return PV{pos[b], vin[b]}; }, [&](u32 b, const PV& r) {
       if (b != i) {
-        float4 pb = pos[b];
-        f3 xcb = xyz(pb) + P.dt * xyz(vin[b]);
+        float4 pb = r.p;
+        f3 xcb = xyz(pb) + P.dt * xyz(r.v);
         f3 dxv = xci - xcb;
         if (dot(dxv, dxv) < P.supp2) {
           float mb = T.mass[matOfTag(pb.w)];
@@ -239,17 +244,18 @@ __global__ __launch_bounds__(256) void k_pbfB(const KP* __restrict__ kps, WorkTa
     f3 dp = splat3(0.0f);
     const float li = aux[i].w;
     const bool iAmWater = !((T.solidMask >> mi) & 1u);
-    forRows(P, cs, cellCoord(P, xn), [&](u32 b) {
+    forRowsG<8>(P, cs, cellCoord(P, xn), [&](u32 b) { return PVL{pos[b], vin[b], aux[b].w}; },
+                [&](u32 b, const PVL& r) {
       if (b != i) {
-        float4 pb = pos[b];
+        float4 pb = r.p;
         u32 mb = matOfTag(pb.w);
         bool bWater = !((T.solidMask >> mb) & 1u);
         if (iAmWater || bWater) {
-          f3 xcb = xyz(pb) + P.dt * xyz(vin[b]);
+          f3 xcb = xyz(pb) + P.dt * xyz(r.v);
           f3 dxv = xci - xcb;
           if (dot(dxv, dxv) < P.supp2) {
             u32 wk = bWater ? mb : mi;
-            float lb = aux[b].w;
+            float lb = r.l;
             float w = kernelPoly6(P, len(dxv));
             float ratio = w / wmax(P.w0, 1e-12f);
             float r2 = ratio * ratio;

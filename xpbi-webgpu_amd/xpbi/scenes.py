@@ -303,16 +303,24 @@ def config_c3(seed=3, dt=1.0 / 6000.0, iterations=10):
     return _finish(s, out, mats, seed, True, info="dam breach 600k water + 400k sand")
 
 
-def config_c5(seed=5, dt=1.0 / 6000.0, iterations=20):
-    """von Mises block 100^3 at r = 0.005 (sigY 5e3, E 2e5, nu 0.3) with a 0.05 m sphere at 1 m/s."""
-    r = 0.005
+def vm_poke(n, r, seed, dt=1.0 / 6000.0, iterations=20, sigY=5e3):
+    """A von Mises block of n^3 particles at spacing r (sigmaY 5e3, E 2e5, nu 0.3) a
+    little above the floor of the unit box, and a sphere of radius 10 r moving down at
+    1 m/s from just above it."""
     s = _solver(r, dt, iterations)
     s["domainMin"], s["domainMax"] = [0.0, 0.0, 0.0], [1.0, 1.0, 1.0]
     out, rng = _new(), Rng(seed, 0)
-    sample_box(out, [0.25, 0.05, 0.25], [0.75, 0.55, 0.75], r, 0, rng, 0.1)
-    s["collider"] = dict(pos=[0.5, 0.62, 0.5], R=0.05, vel=[0.0, -1.0, 0.0])
-    return _finish(s, out, [PLASTICINE(5e3, E=2e5, nu=0.3, density=2000.0)], seed, False,
-                   info="von Mises block 100^3 + poke")
+    e = n * r
+    mn = [0.5 - e / 2, 0.05, 0.5 - e / 2]
+    sample_box(out, mn, [mn[0] + e, mn[1] + e, mn[2] + e], r, 0, rng, 0.1)
+    s["collider"] = dict(pos=[0.5, mn[1] + e + 14 * r, 0.5], R=10 * r, vel=[0.0, -1.0, 0.0])
+    return _finish(s, out, [PLASTICINE(sigY, E=2e5, nu=0.3, density=2000.0)], seed, False,
+                   info=f"von Mises block {n}^3 + poke")
+
+
+def config_c5(seed=5, dt=1.0 / 6000.0, iterations=20):
+    """von Mises block 100^3 at r = 0.005 (sigY 5e3, E 2e5, nu 0.3) with a 0.05 m sphere at 1 m/s."""
+    return vm_poke(100, 0.005, seed, dt, iterations)
 
 
 def config_c4(seed=4, count=64, dt=1.0 / 6000.0, iterations=20, r=0.01):

@@ -22,22 +22,44 @@ from .sim import Simulation, params_from_scene
 WEIGHTS_DIR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "weights")
 
 
+def moving_collider(scene, sim):
+    """fn(t) placing the scene's kinematic sphere at pos + vel t (None without one)."""
+    col = scene.get("collider")
+    if not col or not any(col.get("vel", [0, 0, 0])):
+        return None
+
+    def place(t):
+        pos = [col["pos"][j] + col["vel"][j] * t for j in range(3)]
+        sim.writeParams(params_from_scene(scene, collider=dict(pos=pos, R=col["R"], vel=col["vel"])))
+    return place
+
+
 def collect(scene, settle, substeps, tol=1e-4, cap=200, keep=0.25, seed=0, device=0):
     """Run `settle` plain substeps, then `substeps` more (plain Jacobi to `tol`),
-    keeping a random fraction `keep` of each substep's active pairs.
+    keeping a random fraction `keep` of each substep's active pairs.  A scene with a
+    moving collider gets its position updated every substep.
     Returns features (n x 12), targets (n x 2) and weights (n)."""
     sim = Simulation(scene, device=device)
     sim.writeParams(params_from_scene(scene))
     I, water = scene["iterations"], scene["hasWater"]
-    if settle:
-        sim.step(settle, I, water)
+    place = moving_collider(scene, sim)
+    dt = scene.get("dt") or (1.0 / 60.0) * scene["timeScale"] / scene["substeps"]
+    k = 0
+    for _ in range(settle):
+        if place:
+            place(k * dt)
+        sim.substep(I, water)
+        k += 1
     sim.set_tolerance(tol, 4)
     rng = np.random.default_rng(seed)
     xs, ts, ws = [], [], []
     for _ in range(substeps):
+        if place:
+            place(k * dt)
+        k += 1
         x, t, w = sim.collect_warmstart_pairs(cap, water)
-        # the warm start clamps rho to +-8: targets beyond it cannot be reached
-        ok = np.nonzero(np.isfinite(t[:, 0]) & (np.abs(t[:, 0]) <= 8.0) & (w > 0))[0]
+        # the warm start clamps rho to [0, 2]: targets outside cannot be reached
+        ok = np.nonzero(np.isfinite(t[:, 0]) & (t[:, 0] >= 0.0) & (t[:, 0] <= 2.0) & (w > 0))[0]
         if keep < 1.0:
             ok = rng.choice(ok, size=max(1, int(len(ok) * keep)), replace=False) if len(ok) else ok
         xs.append(x[ok])
@@ -48,12 +70,14 @@ def collect(scene, settle, substeps, tol=1e-4, cap=200, keep=0.25, seed=0, devic
 
 
 def training_columns():
-    """The training scenes: DP sand columns (phi 30) collapsing on the floor at the
-    two resolutions of the 128k and 1M configurations, with their own seeds --
-    (name, scene, settle substeps, collected substeps, kept fraction)."""
+    """The training scenes -- (name, scene, settle substeps, collected substeps, kept
+    fraction): DP sand columns (phi 30) collapsing on the floor at the two resolutions
+    of the 128k and 1M configurations, and a von Mises block poked by a moving sphere
+    (125k particles at r = 0.01, unlike the 1M poke benchmark), each with its own seed."""
     return [("DP sand column 40x80x40 r=0.01 (128,000)", scenes.config_c2(), 300, 40, 0.25),
             ("DP sand column 40x80x40 r=0.005 (128,000)",
-             scenes.sand_column(40, 80, 40, 0.005, [0.75, 0.5, 0.75], seed=7), 420, 40, 0.25)]
+             scenes.sand_column(40, 80, 40, 0.005, [0.75, 0.5, 0.75], seed=7), 420, 40, 0.25),
+            ("von Mises block 50^3 r=0.01 + poke (125,000)", scenes.vm_poke(50, 0.01, seed=9), 440, 40, 0.25)]
 
 
 def training_set(device=0, seed=0, log=None, plan=None):

@@ -259,6 +259,7 @@ int xpbi_create_batch(const xpbi_scene_desc* descs, int count, xpbi_sim** out) {
   A(dalloc(s, &s->selfDv, N));
   A(dalloc(s, &s->nbrIdx, (size_t)s->nbrK * N));
   A(dalloc(s, &s->nbrCnt, N));
+  A(dalloc(s, &s->xc, N));
   A(dalloc(s, &s->cellStart, (size_t)C + 1));
   A(dalloc(s, &s->count, C));
   size_t np = scan_partials_needed(C);

@@ -102,7 +102,8 @@ struct Sim {
   // nbrIdx[k * N + i] for k < nbrCnt[i] <= nbrK (nbrCnt[i] > nbrK: use the cell rows)
   u32* nbrIdx = nullptr;
   u32* nbrCnt = nullptr;
-  u32 nbrK = 64;
+  u32 nbrK = 128;  // 512 B per particle: compressed water reaches ~110 neighbours (q99)
+  float4* xc = nullptr;  // water scenes: predicted positions x + dt v (w = tag) after each solveB
   bool sorted = false;  // a substep has run (cellStart/perm valid)
 
   // residual monitor

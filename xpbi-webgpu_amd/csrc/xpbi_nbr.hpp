@@ -424,9 +424,10 @@ __device__ __forceinline__ void solveBGatherGlobal(const KP& P, const MatLds& T,
               });
 }
 
-// simkernels.js:357-365: impulse cap, contact relaxation, boundaries
+// simkernels.js:357-365: impulse cap, contact relaxation, boundaries; with xcOut
+// also the predicted position x + dt v the PBF sweeps of the iteration test
 __device__ __forceinline__ void solveBFinish(const KP& P, const MatLds& T, u32 i, float4 pi, float4 vi4, f3 dv,
-                                             f3 corr, float4* __restrict__ vout) {
+                                             f3 corr, float4* __restrict__ vout, float4* __restrict__ xcOut) {
   const u32 mi = matOfTag(pi.w);
   const float mass_i = T.mass[mi];
   const f3 xi = xyz(pi);
@@ -436,6 +437,10 @@ __device__ __forceinline__ void solveBFinish(const KP& P, const MatLds& T, u32 i
   f3 v = xyz(vi4) + dvv + corr * (P.contactOmega / P.dt);
   v = applyBoundary(P, xi + P.dt * v, v, xi, T.m[mi].boundFriction);
   vout[i] = make_float4(v.x, v.y, v.z, vi4.w);
+  if (xcOut) {
+    const f3 xc = xi + P.dt * v;
+    xcOut[i] = make_float4(xc.x, xc.y, xc.z, pi.w);
+  }
 }
 
 }  // namespace xpbi

@@ -156,7 +156,7 @@ __global__ __launch_bounds__(256) void k_solveB(const KP* __restrict__ kps, Work
                                                 const float4* __restrict__ vin, M3C Sp,
                                                 const float4* __restrict__ selfDv, const u32* __restrict__ cs,
                                                 NLc nl, float4* __restrict__ vout, const SolveCtl* __restrict__ ctl,
-                                                u32 k) {
+                                                u32 k, float4* __restrict__ xcOut) {
   if (skipIteration(ctl, k)) return;
   __shared__ MatLds T;
   u32 curScene = ~0u;
@@ -171,20 +171,20 @@ __global__ __launch_bounds__(256) void k_solveB(const KP* __restrict__ kps, Work
     f3 dv = splat3(0.0f), corr = splat3(0.0f);
     if ((T.solidMask >> mi) & 1u)
       solveBGatherGlobal(P, T, i, pi, vi4, pos, vin, Sp, selfDv, cs, nl, &dv, &corr);
-    solveBFinish(P, T, i, pi, vi4, dv, corr, vout);
+    solveBFinish(P, T, i, pi, vi4, dv, corr, vout, xcOut);
   });
 }
 
 // ---------------------------------------------------------------- PBF
-// a candidate of pbfB: position/tag, velocity, PBF multiplier
-struct PVL {
-  float4 p, v;
+// a candidate of pbfB: predicted position/tag, PBF multiplier
+struct XL {
+  float4 x;
   float l;
 };
 // simkernels.js:407-439
 template <bool kPersist>
 __global__ __launch_bounds__(256) void k_pbfA(const KP* __restrict__ kps, WorkTab W, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
-                                              const float4* __restrict__ vin, float4* __restrict__ aux,
+                                              const float4* __restrict__ xc, float4* __restrict__ aux,
                                               const u32* __restrict__ cs, const SolveCtl* __restrict__ ctl, u32 k) {
   if (skipIteration(ctl, k)) return;
   __shared__ MatLds T;
@@ -199,17 +199,16 @@ __global__ __launch_bounds__(256) void k_pbfA(const KP* __restrict__ kps, WorkTa
     if ((T.solidMask >> mi) & 1u) { aux[i].w = 0.0f; return; }
     const float rho0 = T.m[mi].density;
     const f3 xn = xyz(pi);
-    const f3 xci = xn + P.dt * xyz(vin[i]);
+    const f3 xci = xyz(xc[i]);
     float rho = T.mass[mi] * P.poly6_0;
     f3 gradSum = splat3(0.0f);
     float grad2 = 0.0f;
-    forRowsG<8>(P, cs, cellCoord(P, xn), [&](u32 b) { return PV{pos[b], vin[b]}; }, [&](u32 b, const PV& r) {
+    // candidates' predicted positions (x + dt v, tag) from the iteration's solveB
+    forRowsG<8>(P, cs, cellCoord(P, xn), [&](u32 b) { return xc[b]; }, [&](u32 b, const float4& xb) {
       if (b != i) {
-        float4 pb = r.p;
-        f3 xcb = xyz(pb) + P.dt * xyz(r.v);
-        f3 dxv = xci - xcb;
+        f3 dxv = xci - xyz(xb);
         if (dot(dxv, dxv) < P.supp2) {
-          float mb = T.mass[matOfTag(pb.w)];
+          float mb = T.mass[matOfTag(xb.w)];
           rho = rho + mb * kernelPoly6(P, len(dxv));
           f3 g = kernelSpikyGrad(P, dxv) * (mb / rho0);
           gradSum = gradSum + g;
@@ -225,9 +224,9 @@ __global__ __launch_bounds__(256) void k_pbfA(const KP* __restrict__ kps, WorkTa
 // simkernels.js:441-477
 template <bool kPersist>
 __global__ __launch_bounds__(256) void k_pbfB(const KP* __restrict__ kps, WorkTab W, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
-                                              const float4* __restrict__ vin, const float4* __restrict__ aux,
-                                              const u32* __restrict__ cs, float4* __restrict__ vout,
-                                              const SolveCtl* __restrict__ ctl, u32 k) {
+                                              const float4* __restrict__ vin, const float4* __restrict__ xc,
+                                              const float4* __restrict__ aux, const u32* __restrict__ cs,
+                                              float4* __restrict__ vout, const SolveCtl* __restrict__ ctl, u32 k) {
   if (skipIteration(ctl, k)) return;
   __shared__ MatLds T;
   u32 curScene = ~0u;
@@ -240,19 +239,16 @@ __global__ __launch_bounds__(256) void k_pbfB(const KP* __restrict__ kps, WorkTa
     const u32 mi = matOfTag(pi.w);
     const f3 xn = xyz(pi);
     const float4 vi4 = vin[i];
-    const f3 xci = xn + P.dt * xyz(vi4);
+    const f3 xci = xyz(xc[i]);
     f3 dp = splat3(0.0f);
     const float li = aux[i].w;
     const bool iAmWater = !((T.solidMask >> mi) & 1u);
-    forRowsG<8>(P, cs, cellCoord(P, xn), [&](u32 b) { return PVL{pos[b], vin[b], aux[b].w}; },
-                [&](u32 b, const PVL& r) {
+    forRowsG<8>(P, cs, cellCoord(P, xn), [&](u32 b) { return XL{xc[b], aux[b].w}; }, [&](u32 b, const XL& r) {
       if (b != i) {
-        float4 pb = r.p;
-        u32 mb = matOfTag(pb.w);
+        u32 mb = matOfTag(r.x.w);
         bool bWater = !((T.solidMask >> mb) & 1u);
         if (iAmWater || bWater) {
-          f3 xcb = xyz(pb) + P.dt * xyz(r.v);
-          f3 dxv = xci - xcb;
+          f3 dxv = xci - xyz(r.x);
           if (dot(dxv, dxv) < P.supp2) {
             u32 wk = bWater ? mb : mi;
             float lb = r.l;
@@ -511,16 +507,16 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
       s->prof_mark("mlp");
       XPBI_LAUNCH_W(k_solveB, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
                          (const float4*)s->vel[vi], cm(s->Sp), (const float4*)s->selfDv, (const u32*)s->cellStart,
-                         nl, s->vel[vi ^ 1], (const SolveCtl*)nullptr, 0u);
+                         nl, s->vel[vi ^ 1], (const SolveCtl*)nullptr, 0u, water ? s->xc : (float4*)nullptr);
       s->prof_mark("solveB");
       vi ^= 1;
       if (water) {
         XPBI_LAUNCH_W(k_pbfA, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
-                           (const float4*)s->vel[vi], S.aux, (const u32*)s->cellStart, (const SolveCtl*)nullptr, 0u);
+                           (const float4*)s->xc, S.aux, (const u32*)s->cellStart, (const SolveCtl*)nullptr, 0u);
         s->prof_mark("pbfA");
         XPBI_LAUNCH_W(k_pbfB, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
-                           (const float4*)s->vel[vi], (const float4*)S.aux, (const u32*)s->cellStart,
-                           s->vel[vi ^ 1], (const SolveCtl*)nullptr, 0u);
+                           (const float4*)s->vel[vi], (const float4*)s->xc, (const float4*)S.aux,
+                           (const u32*)s->cellStart, s->vel[vi ^ 1], (const SolveCtl*)nullptr, 0u);
         s->prof_mark("pbfB");
         vi ^= 1;
       }
@@ -547,16 +543,16 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
     }
     XPBI_LAUNCH_W(k_solveB, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
                        (const float4*)s->vel[vi], cm(s->Sp), (const float4*)s->selfDv, (const u32*)s->cellStart,
-                       nl, s->vel[vi ^ 1], (const SolveCtl*)ctl, (u32)k);
+                       nl, s->vel[vi ^ 1], (const SolveCtl*)ctl, (u32)k, water ? s->xc : (float4*)nullptr);
     s->prof_mark("solveB");
     vi ^= 1;
     if (water) {
       XPBI_LAUNCH_W(k_pbfA, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
-                         (const float4*)s->vel[vi], S.aux, (const u32*)s->cellStart, (const SolveCtl*)ctl, (u32)k);
+                         (const float4*)s->xc, S.aux, (const u32*)s->cellStart, (const SolveCtl*)ctl, (u32)k);
       s->prof_mark("pbfA");
       XPBI_LAUNCH_W(k_pbfB, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
-                         (const float4*)s->vel[vi], (const float4*)S.aux, (const u32*)s->cellStart, s->vel[vi ^ 1],
-                         (const SolveCtl*)ctl, (u32)k);
+                         (const float4*)s->vel[vi], (const float4*)s->xc, (const float4*)S.aux,
+                         (const u32*)s->cellStart, s->vel[vi ^ 1], (const SolveCtl*)ctl, (u32)k);
       s->prof_mark("pbfB");
       vi ^= 1;
     }

@@ -18,6 +18,8 @@ SOURCES = ["xpbi_sort.hip", "xpbi_solve.hip", "xpbi_mlp.hip", "xpbi_api.hip"]
 LIB = os.path.join(PKG, "libxpbi.so")
 ADDON_SRC = os.path.join(PKG, "js", "addon", "xpbi_napi.cc")
 ADDON = os.path.join(PKG, "js", "xpbi_napi.node")
+BENCH_SRC = os.path.join(CSRC, "xpbi_bench_main.cpp")
+BENCH = os.path.join(PKG, "xpbi_bench")
 
 HIPCC_FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
                "-fhip-fp32-correctly-rounded-divide-sqrt",
@@ -57,9 +59,20 @@ def build_addon():
     return ADDON
 
 
+def build_bench():
+    """The native substep driver (C ABI only), linked against libxpbi.so."""
+    tmp = BENCH + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(REPO, "include"), BENCH_SRC, "-L" + PKG, "-lxpbi",
+           "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    _run(cmd)
+    os.replace(tmp, BENCH)
+    return BENCH
+
+
 def build():
     build_lib()
     build_addon()
+    build_bench()
 
 
 if __name__ == "__main__":

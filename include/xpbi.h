@@ -107,7 +107,9 @@ enum {
   XPBI_FIELD_CELLSTART = 6, /* 4*numCells: exclusive start offsets (cellTab after scatter); read only */
   XPBI_FIELD_PERM = 7,      /* 4N: sortedIdx of the last sort; read only */
   XPBI_FIELD_ID = 8,        /* 4N: persistent particle id, permuted with the particles */
-  XPBI_FIELD_COUNT = 9
+  XPBI_FIELD_MATS = 9,      /* 512 per scene: the 8 64-B material records each scene's kernels
+                               read (matsBuf, web/js/sim.js:49; MaterialR, render.js:16-21); read only */
+  XPBI_FIELD_COUNT = 10
 };
 
 /* Stages for xpbi_substep_partial (staged parity checks); XPBI_STAGE_FULL is a
@@ -204,6 +206,24 @@ int xpbi_write(xpbi_sim* sim, int field, const void* src, size_t bytes);
 
 /* Wait for all enqueued work of the sim. */
 int xpbi_sync(xpbi_sim* sim);
+
+/* The device buffers a renderer binds (Renderer.setScene, web/js/main.js:189-199:
+ * posBuf, matsBuf and velA), for zero-copy interop.  pos and vel are N float4
+ * (XPBI_FIELD_POS / _VEL layouts), mats is 8 material records per scene
+ * (XPBI_FIELD_MATS), stream the hipStream_t the sim enqueues on: a consumer
+ * orders its reads after the sim's work on it (or after xpbi_sync).  pos and
+ * vel move between two buffers as the particles are re-sorted, so the view is
+ * valid until the next call that steps, writes or destroys the sim. */
+typedef struct xpbi_render_view {
+  const void* pos;
+  const void* vel;
+  const void* mats;
+  void* stream;
+  uint32_t N;
+  uint32_t scenes;
+  int device;
+} xpbi_render_view;
+int xpbi_get_render_view(xpbi_sim* sim, xpbi_render_view* out);
 
 /* profileSubstep(iterations, hasWater): web/js/sim.js:199-215.  Runs one
  * substep with a HIP event pair around every kernel launch; writes up to cap

@@ -133,6 +133,7 @@ size_t field_bytes(const Sim* s, int field) {
     case XPBI_FIELD_CELLSTART: return 4ull * s->C;
     case XPBI_FIELD_PERM:
     case XPBI_FIELD_ID: return 4ull * s->N;
+    case XPBI_FIELD_MATS: return sizeof(DevMat) * 8ull * s->sc.size();
     default: return 0;
   }
 }
@@ -257,8 +258,8 @@ int xpbi_create_batch(const xpbi_scene_desc* descs, int count, xpbi_sim** out) {
   A(dalloc(s, &s->Sp.b, N));
   A(dalloc(s, &s->Sp.c, N));
   A(dalloc(s, &s->selfDv, N));
-  A(dalloc(s, &s->nbrIdx, (size_t)s->nbrK * N));
-  A(dalloc(s, &s->nbrCnt, N));
+  A(dalloc(s, &s->nbrLst, (size_t)s->nbrK * N));
+  A(dalloc(s, &s->nbrHead, N));
   A(dalloc(s, &s->xc, N));
   A(dalloc(s, &s->cellStart, (size_t)C + 1));
   A(dalloc(s, &s->count, C));
@@ -474,6 +475,7 @@ int xpbi_read(xpbi_sim* h, int field, void* dst, size_t bytes) {
     case XPBI_FIELD_CELLSTART: return d2h(s, dst, s->cellStart, need);
     case XPBI_FIELD_PERM: return d2h(s, dst, s->perm, need);
     case XPBI_FIELD_ID: return d2h(s, dst, S.id, need);
+    case XPBI_FIELD_MATS: return d2h(s, dst, s->dMats, need);
     case XPBI_FIELD_F:
     case XPBI_FIELD_L:
     case XPBI_FIELD_SP: {
@@ -522,6 +524,21 @@ int xpbi_write(xpbi_sim* h, int field, const void* src, size_t bytes) {
     }
     default: return fail(s, XPBI_ERR_ARG, "field is read only");
   }
+}
+
+int xpbi_get_render_view(xpbi_sim* h, xpbi_render_view* out) {
+  if (!h) return XPBI_ERR_ARG;
+  Sim* s = &h->s;
+  if (!out) return fail(s, XPBI_ERR_ARG, "out is null");
+  const PSet& S = s->set[s->cur];
+  out->pos = S.pos;
+  out->vel = s->vel[s->vcur];
+  out->mats = s->dMats;
+  out->stream = (void*)s->stream;
+  out->N = s->N;
+  out->scenes = (uint32_t)s->sc.size();
+  out->device = s->dev;
+  return XPBI_OK;
 }
 
 int xpbi_profile_substep(xpbi_sim* h, int iterations, int has_water, float* ms, int cap, int* count) {

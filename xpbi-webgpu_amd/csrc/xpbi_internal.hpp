@@ -99,10 +99,12 @@ struct Sim {
   u32* scanPart = nullptr;
   u32* radixHist = nullptr;
   // support-neighbour list of the current substep, built by computeL:
-  // nbrIdx[k * N + i] for k < nbrCnt[i] <= nbrK (nbrCnt[i] > nbrK: use the cell rows)
-  u32* nbrIdx = nullptr;
-  u32* nbrCnt = nullptr;
-  u32 nbrK = 128;  // 512 B per particle: compressed water reaches ~110 neighbours (q99)
+  // support-neighbour list: nbrLst[k * N + i] for k < nbrHead[i].y <= nbrK, each
+  // a u16 offset from the base nbrHead[i].x (nbrHead[i].y > nbrK: the sweeps use
+  // the cell rows)
+  uint16_t* nbrLst = nullptr;
+  uint2* nbrHead = nullptr;
+  u32 nbrK = 128;  // 256 B per particle: compressed water reaches ~110 neighbours (q99)
   float4* xc = nullptr;  // water scenes: predicted positions x + dt v (w = tag) after each solveB
   bool sorted = false;  // a substep has run (cellStart/perm valid)
 

@@ -65,38 +65,21 @@ __device__ __forceinline__ void forRowsG(const KP& P, const u32* __restrict__ cs
   });
 }
 
+// Support-neighbour list.  Entry k of particle i is lst[k * N + i], the u16
+// offset b - head[i].x of neighbour b from the particle's first candidate
+// (cellStart at the stencil's lowest cell (x-1, y-1, z-1), clamped to the grid):
+// the candidates of a particle span two z-layers of cells and a little more.
+// head[i].y is the neighbour count; head[i].y > K (more than K neighbours, or
+// kNbrFlag set for an offset past 16 bits) makes the particle's sweeps walk the
+// cell rows.
+typedef uint16_t u16;
+constexpr u32 kNbrFlag = 0x80000000u;
+constexpr u32 kNbrOffMax = 0xffffu;
 struct NLc {
-  const u32* __restrict__ idx;  // idx[k * N + i]
-  const u32* __restrict__ cnt;
+  const uint2* __restrict__ head;
+  const u16* __restrict__ lst;
   u32 K, N;
 };
-
-// fn(b, pos[b], dxv = x_i - x_b, |dxv|^2) for each support neighbour of i.  A
-// particle whose list overflowed (cnt > K) walks the cell rows instead.
-template <class Fn>
-__device__ __forceinline__ void forSupport(const KP& P, const u32* __restrict__ cs, const float4* __restrict__ pos,
-                                           const NLc& nl, u32 i, f3 xi, Fn&& fn) {
-  const u32 n = nl.cnt[i];
-  if (n <= nl.K) {
-    const u32* __restrict__ p = nl.idx + i;
-#pragma unroll 2
-    for (u32 k = 0; k < n; k++) {
-      const u32 b = p[(size_t)k * nl.N];
-      const float4 pb = pos[b];
-      const f3 d = xi - xyz(pb);
-      fn(b, pb, d, dot(d, d));
-    }
-  } else {
-    forRows(P, cs, cellCoord(P, xi), [&](u32 b) {
-      if (b != i) {
-        const float4 pb = pos[b];
-        const f3 d = xi - xyz(pb);
-        const float r2 = dot(d, d);
-        if (r2 < P.supp2) fn(b, pb, d, r2);
-      }
-    });
-  }
-}
 
 #ifndef XPBI_GATHER_BATCH
 #define XPBI_GATHER_BATCH 8
@@ -116,24 +99,28 @@ constexpr int kBatchB = XPBI_GATHER_BATCH_B;
 template <int kB = kBatch, class Load, class Use>
 __device__ __forceinline__ void forSupportG(const KP& P, const u32* __restrict__ cs, const NLc& nl, u32 i, f3 xi,
                                             Load&& load, Use&& use) {
-  const u32 n = nl.cnt[i];
+  const uint2 hd = nl.head[i];
+  const u32 n = hd.y;
   if (n <= nl.K) {
-    const u32* __restrict__ p = nl.idx + i;
-    u32 bn[kB];
+    const u16* __restrict__ p = nl.lst + i;
+    // raw entries of the next batch; the base is added when the batch is used, so
+    // that no arithmetic on a loaded entry sits inside the load's predicate
+    // (which would wait on every load in flight)
+    u32 en[kB];
 #pragma unroll
-    for (int j = 0; j < kB; j++) bn[j] = ((u32)j < n) ? p[(size_t)j * nl.N] : i;
+    for (int j = 0; j < kB; j++) en[j] = ((u32)j < n) ? (u32)p[(size_t)j * nl.N] : 0u;
     for (u32 k = 0; k < n; k += kB) {
       u32 b[kB];
       decltype(load(0u)) d[kB];
 #pragma unroll
       for (int j = 0; j < kB; j++) {
-        b[j] = bn[j];
+        b[j] = (k + (u32)j < n) ? hd.x + en[j] : i;
         d[j] = load(b[j]);
       }
 #pragma unroll
       for (int j = 0; j < kB; j++) {
         u32 kk = k + kB + (u32)j;
-        bn[j] = (kk < n) ? p[(size_t)kk * nl.N] : i;
+        en[j] = (kk < n) ? (u32)p[(size_t)kk * nl.N] : 0u;
       }
 #pragma unroll
       for (int j = 0; j < kB; j++) {

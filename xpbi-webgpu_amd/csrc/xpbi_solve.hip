@@ -20,48 +20,77 @@ namespace xpbi {
 // support-neighbour list of the substep.
 template <bool kPersist>
 __global__ __launch_bounds__(256) void k_computeL(const KP* __restrict__ kps, WorkTab W, const float4* __restrict__ pos, const float4* __restrict__ vin,
-                                                  const u32* __restrict__ cs, M3P L, u32* __restrict__ nIdx,
-                                                  u32* __restrict__ nCnt, u32 K, u32 NT) {
+                                                  const u32* __restrict__ cs, M3P L, u16* __restrict__ nLst,
+                                                  uint2* __restrict__ nHead, u32 K, u32 NT) {
+  // each thread's first kStage list entries, written out slot by slot at the
+  // end so that a store covers consecutive particles (entries found during the
+  // walk land in different slots per lane)
+  constexpr u32 kStage = 64;
+  __shared__ u16 stage[kStage][256];
+  const u32 tid = threadIdx.x;
   forWork<kPersist>(W, [&](u32 scene, u32 first) {
     const KP& P = kps[scene];
-    const u32 i = first + threadIdx.x;
+    const u32 i = first + tid;
     if (i >= P.pEnd) return;
     f3 xi = xyz(pos[i]);
     f33 A = mzero<float>();
-    u32 n = 0;
-    // candidates in batches of 8: their positions, then the volumes of the
-    // accepted ones, each batch's loads issued together; accumulation and the
-    // list keep the candidate order
-    forRowSpans(P, cs, cellCoord(P, xi), [&](u32 beg, u32 end) {
-      for (u32 b0 = beg; b0 < end; b0 += 8u) {
-        float4 pb[8];
+    u32 n = 0, flag = 0;
+    // the 9 row spans of forRowSpans, entries relative to the first candidate
+    // (the lowest cell of the stencil); candidates in batches of 8: their
+    // positions, then the volumes of the accepted ones, each batch's loads
+    // issued together; accumulation and the list keep the candidate order
+    const i3 cc = cellCoord(P, xi);
+    const u32 x0 = (u32)max(cc.x - 1, 0), x1 = (u32)min(cc.x + 1, P.gx - 1);
+    const u32 y0 = (u32)max(cc.y - 1, 0), z0 = (u32)max(cc.z - 1, 0);
+    const u32 base = cs[P.cellBase + z0 * P.gxy + y0 * (u32)P.gx + x0];
+#pragma unroll 1
+    for (int dz = -1; dz <= 1; dz++) {
+      const int z = cc.z + dz;
+      if (z < 0 || z >= P.gz) continue;
+      const u32 plane = P.cellBase + (u32)z * P.gxy;
+#pragma unroll 1
+      for (int dy = -1; dy <= 1; dy++) {
+        const int y = cc.y + dy;
+        if (y < 0 || y >= P.gy) continue;
+        const u32 row = plane + (u32)y * (u32)P.gx;
+        const u32 beg = cs[row + x0];
+        const u32 end = min(cs[row + x1 + 1u], beg + kRowCap);
+        for (u32 b0 = beg; b0 < end; b0 += 8u) {
+          float4 pb[8];
 #pragma unroll
-        for (int j = 0; j < 8; j++) pb[j] = (b0 + j < end) ? pos[b0 + j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        f3 d[8];
-        float r2[8];
-        bool ok[8];
+          for (int j = 0; j < 8; j++) pb[j] = (b0 + j < end) ? pos[b0 + j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+          f3 d[8];
+          float r2[8];
+          bool ok[8];
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-          d[j] = xi - xyz(pb[j]);
-          r2[j] = dot(d[j], d[j]);
-          ok[j] = (b0 + j < end) && (b0 + j != i) && (r2[j] < P.supp2);
-        }
-        float Vb[8];
+          for (int j = 0; j < 8; j++) {
+            d[j] = xi - xyz(pb[j]);
+            r2[j] = dot(d[j], d[j]);
+            ok[j] = (b0 + j < end) && (b0 + j != i) && (r2[j] < P.supp2);
+          }
+          float Vb[8];
 #pragma unroll
-        for (int j = 0; j < 8; j++) Vb[j] = ok[j] ? vin[b0 + j].w : 0.0f;
+          for (int j = 0; j < 8; j++) Vb[j] = ok[j] ? vin[b0 + j].w : 0.0f;
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-          if (ok[j]) {
-            f3 g = gradWfast(P, d[j], r2[j]);
-            f3 r = -d[j];
-            A = madd(A, mscale(outer(g, r), Vb[j]));
-            if (n < K) nIdx[(size_t)n * NT + i] = b0 + j;
-            n++;
+          for (int j = 0; j < 8; j++) {
+            if (ok[j]) {
+              f3 g = gradWfast(P, d[j], r2[j]);
+              f3 r = -d[j];
+              A = madd(A, mscale(outer(g, r), Vb[j]));
+              const u32 off = b0 + (u32)j - base;
+              flag |= off > kNbrOffMax ? kNbrFlag : 0u;
+              const u16 e = (u16)off;
+              if (n < kStage) stage[n][tid] = e;
+              else if (n < K) nLst[(size_t)n * NT + i] = e;
+              n++;
+            }
           }
         }
       }
-    });
-    nCnt[i] = n;
+    }
+    const u32 ns = min(n, kStage);
+    for (u32 k = 0; k < ns; k++) nLst[(size_t)k * NT + i] = stage[k][tid];
+    nHead[i] = make_uint2(base, n | flag);
     SVD3<double> s = svd3<double>(mcast<double>(A));
     d3 inv = splat3(0.0);
     if (fabs(s.sig.x) > 0.2) inv.x = 1.0 / s.sig.x;
@@ -417,7 +446,7 @@ __global__ __launch_bounds__(256) void k_features(const KP* __restrict__ kps, Wo
         row[6] = m.lam / (m.lam + 2.0f * m.mu);
         row[7] = (float)m.kind / 6.0f;
         row[8] = strength;
-        row[9] = (float)nl.cnt[i] / 40.0f;
+        row[9] = (float)(nl.head[i].y & ~kNbrFlag) / 40.0f;
         row[10] = (d0 != 0.0f && lamPrev) ? wclamp(lamPrev[i] / d0, -8.0f, 8.0f) : 0.0f;
         row[11] = P.alphaT / den;
         row[12] = d0;
@@ -483,8 +512,8 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
   hipStream_t st = s->stream;
   int vi = s->vcur;
   XPBI_LAUNCH_W(k_computeL, dim3(gN), dim3(256), 0, st, P, W, (const float4*)S.pos, (const float4*)s->vel[vi],
-                     (const u32*)s->cellStart, s->L, s->nbrIdx, s->nbrCnt, s->nbrK, s->N);
-  const NLc nl{s->nbrIdx, s->nbrCnt, s->nbrK, s->N};
+                     (const u32*)s->cellStart, s->L, s->nbrLst, s->nbrHead, s->nbrK, s->N);
+  const NLc nl{s->nbrHead, s->nbrLst, s->nbrK, s->N};
   s->prof_mark("computeL");
   if (stop == XPBI_STAGE_COMPUTE_L) return;
   s->residIters = 0;

@@ -239,6 +239,12 @@ napi_value Read(napi_env env, napi_callback_info info) {
     case XPBI_FIELD_F: case XPBI_FIELD_L: case XPBI_FIELD_SP: bytes = 48ull * np; break;
     case XPBI_FIELD_CELLSTART: bytes = 4ull * nc; break;
     case XPBI_FIELD_PERM: case XPBI_FIELD_ID: bytes = 4ull * np; break;
+    case XPBI_FIELD_MATS: {
+      int scenes = 0;
+      xpbi_scene_count(s, &scenes);
+      bytes = 512ull * (size_t)scenes;
+      break;
+    }
     default: napi_throw_range_error(env, nullptr, "unknown field"); return nullptr;
   }
   void* data = nullptr;

@@ -16,7 +16,7 @@ function loadAddon() {
 
 const MAT_STRIDE = 16; // floats per material (sim.js:4)
 const MAT = { ELASTIC: 0, DP: 1, NACC: 2, VM: 3, HB: 4, SNOW: 5, WATER: 6 };
-const FIELD = { POS: 0, VEL: 1, F: 2, L: 3, AUX: 4, SP: 5, CELLSTART: 6, PERM: 7, ID: 8 };
+const FIELD = { POS: 0, VEL: 1, F: 2, L: 3, AUX: 4, SP: 5, CELLSTART: 6, PERM: 7, ID: 8, MATS: 9 };
 
 function def(v, d) { return v === undefined || v === null ? d : v; }
 

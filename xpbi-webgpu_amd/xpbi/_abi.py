@@ -16,7 +16,7 @@ OK, ERR_ARG, ERR_HIP, ERR_STATE, ERR_NOMEM = 0, -1, -2, -3, -4
 # material kinds (web/js/sim.js:5-7)
 MAT = dict(ELASTIC=0, DP=1, NACC=2, VM=3, HB=4, SNOW=5, WATER=6)
 
-FIELD = dict(POS=0, VEL=1, F=2, L=3, AUX=4, SP=5, CELLSTART=6, PERM=7, ID=8)
+FIELD = dict(POS=0, VEL=1, F=2, L=3, AUX=4, SP=5, CELLSTART=6, PERM=7, ID=8, MATS=9)
 STAGE = dict(SORT=1, REORDER=2, COMPUTE_L=3, SOLVE=4, XSPH=5, UPDATE_F=6, FULL=7)
 
 
@@ -51,6 +51,12 @@ class SceneDesc(C.Structure):
                 ("domainMax", C.c_double * 3), ("device", C.c_int)]
 
 
+class RenderView(C.Structure):
+    """xpbi_render_view: the device buffers a renderer binds (main.js:189-199)."""
+    _fields_ = [("pos", C.c_void_p), ("vel", C.c_void_p), ("mats", C.c_void_p), ("stream", C.c_void_p),
+                ("N", C.c_uint32), ("scenes", C.c_uint32), ("device", C.c_int)]
+
+
 assert C.sizeof(Material) == 64 and C.sizeof(Params) == 256
 
 # every symbol include/xpbi.h declares, with (restype, argtypes)
@@ -69,6 +75,7 @@ SIGNATURES = {
     "xpbi_read": (C.c_int, [_P, C.c_int, _P, C.c_size_t]),
     "xpbi_write": (C.c_int, [_P, C.c_int, _P, C.c_size_t]),
     "xpbi_sync": (C.c_int, [_P]),
+    "xpbi_get_render_view": (C.c_int, [_P, C.POINTER(RenderView)]),
     "xpbi_profile_substep": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int,
                                        C.POINTER(C.c_int)]),
     "xpbi_profile_label": (C.c_char_p, [_P, C.c_int]),

@@ -11,7 +11,7 @@ import ctypes as C
 import numpy as np
 
 from . import _abi
-from ._abi import FIELD, STAGE, XpbiError, check, lib
+from ._abi import FIELD, STAGE, RenderView, XpbiError, check, lib
 
 
 def _fptr(a):
@@ -113,6 +113,7 @@ class Simulation:
         self.gridOrigin, self.cellSize, self.gridDims, self.numCells = (s0.gridOrigin, s0.cellSize, s0.gridDims,
                                                                         s0.numCells)
         self.totalCells = sum(si.numCells for si in self.scenes_info)
+        self.num_scenes = len(scenes)
         for q, sc in enumerate(scenes):
             self.writeMaterials(sc["materials"], scene=q)
 
@@ -212,6 +213,8 @@ class Simulation:
     def read(self, field):
         if field == "CELLSTART":
             out = np.empty(self.totalCells, dtype=np.uint32)
+        elif field == "MATS":
+            out = np.empty((self.num_scenes, 8, 16), dtype=np.float32)
         else:
             w, dt = self._SHAPES[field]
             out = np.empty((self.N, w) if w > 1 else self.N, dtype=dt)
@@ -221,6 +224,13 @@ class Simulation:
     def write(self, field, arr):
         arr = np.ascontiguousarray(arr)
         check(lib().xpbi_write(self._h, FIELD[field], arr.ctypes.data, arr.nbytes), self._h, f"write {field}")
+
+    def render_view(self):
+        """Device pointers of pos, velA and the material table, and the sim's stream
+        (xpbi_get_render_view); valid until the next step, write or close."""
+        v = RenderView()
+        check(lib().xpbi_get_render_view(self._h, C.byref(v)), self._h, "render_view")
+        return dict(pos=v.pos, vel=v.vel, mats=v.mats, stream=v.stream, N=v.N, scenes=v.scenes, device=v.device)
 
     def readPos(self):
         return self.read("POS")

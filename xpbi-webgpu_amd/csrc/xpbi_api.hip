@@ -220,7 +220,8 @@ int xpbi_create_batch(const xpbi_scene_desc* descs, int count, xpbi_sim** out) {
     Ct += sc.C;
     nbt += (sc.N + 255) / 256;
   }
-  if (Nt >= (1ull << 31) || Ct >= 4294967295ull) return fail(nullptr, XPBI_ERR_ARG, "batch too large");
+  // gathers address 16-B records by 32-bit byte offsets: at most 2^28 particles
+  if (Nt >= (1ull << 28) || Ct >= 4294967295ull) return fail(nullptr, XPBI_ERR_ARG, "batch too large");
 
   xpbi_sim* h = new xpbi_sim();
   Sim* s = &h->s;

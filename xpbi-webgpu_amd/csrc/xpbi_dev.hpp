@@ -225,6 +225,18 @@ __device__ __forceinline__ f3 kernelSpikyGrad(const KP& P, f3 dxv) {
 // common.js:97-103.  IEEE f32 division (hipcc's correctly rounded default) and
 // floor; v_cvt_i32_f32 saturates and maps NaN to 0 like the WGSL conversion.
 struct i3 { int x, y, z; };
+
+// element idx of an array read by a gather, addressed by a 32-bit byte offset
+// from the array's base (idx * sizeof(T) < 2^32: at most 2^28 16-B records), so
+// that the load takes the base in scalar registers and one 32-bit vector offset
+template <class T>
+__device__ __forceinline__ T gat(const T* __restrict__ base, u32 idx) {
+  return *(const T*)((const char*)base + idx * (u32)sizeof(T));
+}
+// the w lane of float4 record idx
+__device__ __forceinline__ float gatW(const float4* __restrict__ base, u32 idx) {
+  return *(const float*)((const char*)base + idx * 16u + 12u);
+}
 __device__ __forceinline__ int f2i_sat(float f) { return __float2int_rz(f); }
 __device__ __forceinline__ i3 cellCoord(const KP& P, f3 p) {
   int cx = f2i_sat(floorf((p.x - P.ox) / P.cell));

@@ -102,25 +102,27 @@ __device__ __forceinline__ void forSupportG(const KP& P, const u32* __restrict__
   const uint2 hd = nl.head[i];
   const u32 n = hd.y;
   if (n <= nl.K) {
-    const u16* __restrict__ p = nl.lst + i;
-    // raw entries of the next batch; the base is added when the batch is used, so
-    // that no arithmetic on a loaded entry sits inside the load's predicate
-    // (which would wait on every load in flight)
+    // raw entries of the next batch (past the count: the particle's own offset,
+    // so a batch's tail gathers i itself); the base is added when the batch is
+    // used, so that no arithmetic on a loaded entry sits inside the load's
+    // predicate (which would wait on every load in flight).  Slot kk's row
+    // nl.lst + kk N is uniform: the entry load is a scalar base plus 2 i.
+    const u32 self = i - hd.x;
     u32 en[kB];
 #pragma unroll
-    for (int j = 0; j < kB; j++) en[j] = ((u32)j < n) ? (u32)p[(size_t)j * nl.N] : 0u;
+    for (int j = 0; j < kB; j++) en[j] = ((u32)j < n) ? (u32)gat(nl.lst + (size_t)j * nl.N, i) : self;
     for (u32 k = 0; k < n; k += kB) {
       u32 b[kB];
       decltype(load(0u)) d[kB];
 #pragma unroll
       for (int j = 0; j < kB; j++) {
-        b[j] = (k + (u32)j < n) ? hd.x + en[j] : i;
+        b[j] = hd.x + en[j];
         d[j] = load(b[j]);
       }
 #pragma unroll
       for (int j = 0; j < kB; j++) {
         u32 kk = k + kB + (u32)j;
-        en[j] = (kk < n) ? (u32)p[(size_t)kk * nl.N] : 0u;
+        en[j] = (kk < n) ? (u32)gat(nl.lst + (size_t)kk * nl.N, i) : self;
       }
 #pragma unroll
       for (int j = 0; j < kB; j++) {
@@ -220,7 +222,7 @@ __device__ __forceinline__ Moments neighbourMoments(const KP& P, const u32* __re
                                                     f3 xi, f3 vi) {
   Moments m;
   momentsZero(m);
-  forSupportG(P, cs, nl, i, xi, [&](u32 b) { return PV{pos[b], vin[b]}; },
+  forSupportG(P, cs, nl, i, xi, [&](u32 b) { return PV{gat(pos, b), gat(vin, b)}; },
               [&](u32 b, const PV& r, f3 dxv, float r2) {
                 const float4 pb = r.p, vb = r.v;
                 if (solidTag(solidMask, pb.w)) {
@@ -404,6 +406,9 @@ __device__ __forceinline__ void solveBGatherGlobal(const KP& P, const MatLds& T,
   const f3 xi = xyz(pi);
   const f3 xci = xi + P.dt * xyz(vi4);
   const float mass_i = T.mass[matOfTag(pi.w)];
+  // indexed by pointer, not by gat(): the compiler then issues a batch's loads
+  // grouped by array, which measured 30% faster for these five arrays than the
+  // per-neighbour interleaving it emits for 32-bit offsets
   forSupportG<kBatchB>(P, cs, nl, i, xi, [&](u32 b) { return SpGather{pos[b], vin[b], Sp.a[b], Sp.b[b], Sp.c[b]}; },
               [&](u32 b, const SpGather& g, f3 dxv, float r2) {
                 const f33 Spb{{f3{g.a.x, g.a.y, g.a.z}, f3{g.a.w, g.b.x, g.b.y}, f3{g.b.z, g.b.w, g.c}}};

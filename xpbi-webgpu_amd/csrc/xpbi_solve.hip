@@ -233,7 +233,7 @@ __global__ __launch_bounds__(256) void k_pbfA(const KP* __restrict__ kps, WorkTa
     f3 gradSum = splat3(0.0f);
     float grad2 = 0.0f;
     // candidates' predicted positions (x + dt v, tag) from the iteration's solveB
-    forRowsG<8>(P, cs, cellCoord(P, xn), [&](u32 b) { return xc[b]; }, [&](u32 b, const float4& xb) {
+    forRowsG<8>(P, cs, cellCoord(P, xn), [&](u32 b) { return gat(xc, b); }, [&](u32 b, const float4& xb) {
       if (b != i) {
         f3 dxv = xci - xyz(xb);
         if (dot(dxv, dxv) < P.supp2) {
@@ -272,7 +272,7 @@ __global__ __launch_bounds__(256) void k_pbfB(const KP* __restrict__ kps, WorkTa
     f3 dp = splat3(0.0f);
     const float li = aux[i].w;
     const bool iAmWater = !((T.solidMask >> mi) & 1u);
-    forRowsG<8>(P, cs, cellCoord(P, xn), [&](u32 b) { return XL{xc[b], aux[b].w}; }, [&](u32 b, const XL& r) {
+    forRowsG<8>(P, cs, cellCoord(P, xn), [&](u32 b) { return XL{gat(xc, b), gatW(aux, b)}; }, [&](u32 b, const XL& r) {
       if (b != i) {
         u32 mb = matOfTag(r.x.w);
         bool bWater = !((T.solidMask >> mb) & 1u);
@@ -315,7 +315,7 @@ __global__ __launch_bounds__(256) void k_xsph(const KP* __restrict__ kps, WorkTa
     const f3 vi = xyz(vi4);
     const bool iSolid = solidTag(T.solidMask, pi.w);
     f3 acc = splat3(0.0f);
-    forSupportG(P, cs, nl, i, xi, [&](u32 b) { return PV{pos[b], vin[b]}; },
+    forSupportG(P, cs, nl, i, xi, [&](u32 b) { return PV{gat(pos, b), gat(vin, b)}; },
                 [&](u32 b, const PV& r, f3 dxv, float r2) {
                   if (solidTag(T.solidMask, r.p.w) == iSolid) acc = acc + r.v.w * (xyz(r.v) - vi) * kernelWfast(P, r2);
                 });

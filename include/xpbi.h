@@ -333,6 +333,44 @@ int xpbi_mlp_adam_step(xpbi_mlp* mlp, const float* grad, float lr);
  * (seeded); epoch_loss (epochs floats, may be NULL) gets each epoch's mean loss. */
 int xpbi_mlp_train(xpbi_mlp* mlp, int epochs, int batch, float lr, uint64_t seed, float* epoch_loss);
 
+/* ---- 2D fp64 engine: the algorithm of prototype/xpbi2d.py:123-241 ---------
+ * The prototype is a script (no API): a sand column whose state is x, v, F
+ * (xpbi2d.py:52-58) advanced by the main loop (xpbi2d.py:123-241) with
+ * constants fixed at xpbi2d.py:29-49.  Here the constants are xpbi2d_params and
+ * the loop is xpbi2d_step.  One workgroup per scene runs a call's substeps in
+ * one launch, in float64. */
+typedef struct xpbi2d_params {
+  double h;           /* spacing = kernel radius (xpbi2d.py:29-30); support 2h, V0 = h^2 */
+  double E, nu, rho;  /* StVK-Hencky moduli and density (xpbi2d.py:32-36) */
+  double phi_deg;     /* Drucker-Prager friction angle (xpbi2d.py:37-38) */
+  double dt;          /* substep (xpbi2d.py:39) */
+  int iterations;     /* Jacobi iterations (xpbi2d.py:40) */
+  double gravity[2];  /* (xpbi2d.py:41) */
+  double xsph_c;      /* (xpbi2d.py:42) */
+  double domain[2];   /* box [0, domain]; walls at h from each side (xpbi2d.py:49, :205-206) */
+  double omega;       /* Delta-lambda multiplier; the prototype's is 1 (xpbi2d.py:178-180) */
+} xpbi2d_params;
+enum {
+  XPBI2D_FIELD_X = 0,      /* 2N doubles: positions */
+  XPBI2D_FIELD_V = 1,      /* 2N doubles: velocities */
+  XPBI2D_FIELD_F = 2,      /* 4N doubles: F row-major (F00, F01, F10, F11) */
+  XPBI2D_FIELD_LAMBDA = 3  /* N doubles: the last substep's lambda; read only */
+};
+#define XPBI2D_MAX_PARTICLES 65536u
+typedef struct xpbi2d_sim xpbi2d_sim;
+/* positions: 2N doubles; v = 0 and F = I initially (xpbi2d.py:57-58). */
+int xpbi2d_create(uint32_t N, const double* positions, const xpbi2d_params* params, int device, xpbi2d_sim** out);
+void xpbi2d_destroy(xpbi2d_sim* sim);
+const char* xpbi2d_last_error(xpbi2d_sim* sim);
+/* substeps iterations of xpbi2d.py:123-237; returns when they are done.
+ * XPBI_ERR_STATE when a particle had more than 48 neighbours within 2h. */
+int xpbi2d_step(xpbi2d_sim* sim, int substeps);
+/* xpbi2d_step with a HIP event pair around the launch: *ms = its duration. */
+int xpbi2d_time(xpbi2d_sim* sim, int substeps, float* ms);
+/* count = the field's size in doubles. */
+int xpbi2d_read(xpbi2d_sim* sim, int field, double* dst, size_t count);
+int xpbi2d_write(xpbi2d_sim* sim, int field, const double* src, size_t count);
+
 #ifdef __cplusplus
 }
 #endif

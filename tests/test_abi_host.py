@@ -24,7 +24,7 @@ JS = os.path.join(REPO, "xpbi-webgpu_amd", "js")
 def header_functions():
     src = open(HEADER).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(xpbi_[a-z_0-9]+)\s*\(", src)))
+    return sorted(set(re.findall(r"\b(xpbi(?:2d)?_[a-z_0-9]+)\s*\(", src)))
 
 
 def test_library_exports_every_declared_symbol():

@@ -1,0 +1,84 @@
+"""The 2D fp64 engine (xpbi2d_*) against the oracle's restatement of
+prototype/xpbi2d.py on the same inputs.
+
+Bar: float64, every sum in the same order; the two may differ only where the
+device's and numpy's log/exp round apart (a few ulp), so positions, velocities, F
+and lambda agree to 1e-11 relative after 20 substeps of the 3,200-particle column
+and after 150 of the prototype's own 666-particle column (chaotic growth of the
+ulp differences stays below that over these spans)."""
+import numpy as np
+import pytest
+
+import proto2d as o
+from xpbi import XpbiError
+from xpbi import proto2d as host
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-11
+
+
+def _oracle_prm(p):
+    return o.make_params(p["h"], p["E"], p["nu"], p["rho"], p["phi_deg"], p["dt"], p["iterations"], p["gravity"],
+                         p["xsph_c"], p["domain"], p["omega"])
+
+
+def _compare(sim, st, what):
+    errs = {}
+    for k, ref in (("X", st["x"]), ("V", st["v"]), ("F", st["F"]), ("LAMBDA", st["lam"])):
+        got = sim.read(k)
+        scale = max(np.abs(ref).max(), 1e-30)
+        errs[k] = float(np.abs(got - ref).max() / scale)
+    print(what, {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
+    for k, v in errs.items():
+        assert v <= TOL, (what, k, v)
+
+
+@pytest.mark.parametrize("case,chunks", [("c1", [1, 4, 15]), ("prototype", [10, 40, 100])])
+def test_engine_matches_oracle(case, chunks):
+    if case == "c1":
+        p, x = host.c1_params(), host.c1_positions()
+    else:
+        p, x = host.prototype_params(), host.prototype_positions()
+    prm = _oracle_prm(p)
+    st = o.initial_state(x)
+    sim = host.Prototype2D(x, p)
+    done = 0
+    for k in chunks:
+        print(f"{case}: substeps {done}..{done + k}", flush=True)
+        sim.step(k)
+        o.run(prm, st, k)
+        done += k
+        _compare(sim, st, f"{case} after {done}")
+    assert np.abs(st["v"]).max() > 1e-3  # the column moved
+    sim.close()
+
+
+def test_rest_lattice_and_momentum_on_device():
+    p = dict(host.c1_params(), domain=[4.0, 4.0], gravity=[0.0, 0.0])
+    ax = np.arange(12) * 0.01 + 1.5
+    X, Y = np.meshgrid(ax, ax)
+    lat = np.stack([X.ravel(), Y.ravel()], 1)
+    sim = host.Prototype2D(lat, p)
+    sim.step(3)
+    assert np.array_equal(sim.x, lat) and not sim.v.any()
+    sim.close()
+    # a disturbed blob away from the walls: internal impulses move no momentum
+    g = np.random.default_rng(8)
+    x = host.column(16, 16, 0.01, 1.5, 1.5, seed=7) + g.normal(0, 0.002, (256, 2))
+    v = g.normal(0, 0.05, (256, 2))
+    p = dict(host.c1_params(), domain=[4.0, 4.0])
+    sim = host.Prototype2D(x, p)
+    sim.write("V", v)
+    sim.step(1)
+    want = v.sum(0) + 256 * p["dt"] * np.asarray(p["gravity"])
+    assert np.max(np.abs(sim.v.sum(0) - want)) < 1e-12 * 256
+    assert np.abs(sim.lam).max() > 0
+    sim.close()
+
+
+def test_neighbour_overflow_is_an_error():
+    x = 0.5 + np.random.default_rng(1).uniform(-0.002, 0.002, (60, 2))  # 60 particles within 2h
+    sim = host.Prototype2D(x, host.c1_params())
+    with pytest.raises(XpbiError, match="neighbours"):
+        sim.step(1)
+    sim.close()

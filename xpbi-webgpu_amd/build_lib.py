@@ -14,7 +14,7 @@ import sys
 PKG = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
-SOURCES = ["xpbi_sort.hip", "xpbi_solve.hip", "xpbi_mlp.hip", "xpbi_api.hip"]
+SOURCES = ["xpbi_sort.hip", "xpbi_solve.hip", "xpbi_mlp.hip", "xpbi_api.hip", "xpbi2d.hip"]
 LIB = os.path.join(PKG, "libxpbi.so")
 ADDON_SRC = os.path.join(PKG, "js", "addon", "xpbi_napi.cc")
 ADDON = os.path.join(PKG, "js", "xpbi_napi.node")

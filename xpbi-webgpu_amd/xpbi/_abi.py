@@ -57,6 +57,15 @@ class RenderView(C.Structure):
                 ("N", C.c_uint32), ("scenes", C.c_uint32), ("device", C.c_int)]
 
 
+class Params2D(C.Structure):
+    """xpbi2d_params: the 2D engine's constants (prototype/xpbi2d.py:29-49)."""
+    _fields_ = [("h", C.c_double), ("E", C.c_double), ("nu", C.c_double), ("rho", C.c_double),
+                ("phi_deg", C.c_double), ("dt", C.c_double), ("iterations", C.c_int), ("gravity", C.c_double * 2),
+                ("xsph_c", C.c_double), ("domain", C.c_double * 2), ("omega", C.c_double)]
+
+
+FIELD2D = dict(X=0, V=1, F=2, LAMBDA=3)
+
 assert C.sizeof(Material) == 64 and C.sizeof(Params) == 256
 
 # every symbol include/xpbi.h declares, with (restype, argtypes)
@@ -113,6 +122,13 @@ SIGNATURES = {
     "xpbi_mlp_loss_grad": (C.c_int, [_P, C.c_size_t, C.c_size_t, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "xpbi_mlp_adam_step": (C.c_int, [_P, C.POINTER(C.c_float), C.c_float]),
     "xpbi_mlp_train": (C.c_int, [_P, C.c_int, C.c_int, C.c_float, C.c_uint64, C.POINTER(C.c_float)]),
+    "xpbi2d_create": (C.c_int, [C.c_uint32, C.POINTER(C.c_double), C.POINTER(Params2D), C.c_int, C.POINTER(_P)]),
+    "xpbi2d_destroy": (None, [_P]),
+    "xpbi2d_last_error": (C.c_char_p, [_P]),
+    "xpbi2d_step": (C.c_int, [_P, C.c_int]),
+    "xpbi2d_time": (C.c_int, [_P, C.c_int, C.POINTER(C.c_float)]),
+    "xpbi2d_read": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double), C.c_size_t]),
+    "xpbi2d_write": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double), C.c_size_t]),
 }
 
 # warm-start network (include/xpbi.h XPBI_MLP_*)

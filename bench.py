@@ -3,21 +3,26 @@
 
 Workload: 1M-particle Drucker-Prager sand -- an 80x160x80 seeded-jitter column
 at r = 0.005 m (1,024,000 particles; E 3.5e5 Pa, nu 0.3, rho 2000, phi 30 deg),
-dt = 1/6000 s, 20 Jacobi iterations.  Before warm-up the column is advanced
-`--settle` substeps (untimed) so that it has landed and deforms; a free-falling
+dt = 1/6000 s.  Before warm-up the column is advanced `--settle` substeps
+(untimed, 20 iterations each) so that it has landed and deforms; a free-falling
 column would skip the constraint sweeps.  A step is one substep of all
-particles; `value` = particles x substeps / wall seconds of the timed region,
-with every array resident in HBM.
+particles, with every array resident in HBM.
 
-The timed region replays the substep from a HIP graph (two captured substeps)
-under the launch shape the autotuner picks.  Also reported: per-kernel HIP-event
-times of an eager run of the same substep, the
-HBM roofline of the dominant kernel (solveA), Jacobi to residual 1e-4 -- plain
-from lambda = 0 against the learned warm start, whose network is trained in the
-run on the GPU (pairs from DP columns of 128,000 particles at r = 0.01 and
-0.005, never the 1M column itself) -- the 64-scene batched ensemble (one launch
-per kernel for all scenes, graph-replayed, autotuned), and the CPU oracle's
-substep on the host cores (a reported baseline).
+`value` = particles x substeps / wall seconds of `--steps` substeps whose Jacobi
+iterations run to residual 1e-4 (cap `--residual-cap`) from the learned warm
+start; `plain_value` is the same from lambda = 0, from the same snapshot, to the
+same residual.  The warm-start network is trained in the run on the GPU (pairs
+from DP columns of 128,000 particles at r = 0.01 and 0.005 and a von Mises poke,
+never the 1M column itself).  With --no-residual, `value` is the fixed
+20-iteration throughput instead.
+
+Also reported: the fixed 20-iteration throughput replayed from a HIP graph under
+the autotuned launch shape; per-kernel HIP-event times of an eager run; the HBM
+roofline of the dominant kernel (solveA, with its HBM traffic from the committed
+counter profile); the 64-scene batched ensemble; the mixed water + sand scene;
+the von Mises poke to residual; the 2D fp64 column of the prototype (config 1)
+beside its numpy oracle; and the f32 C++ oracle's substep on the host cores (a
+reported baseline).
 """
 import argparse
 import json
@@ -80,9 +85,12 @@ def trained_network(weights=None, log=None):
                         collect_s=round(t1 - t0, 2), train_s=round(time.perf_counter() - t1, 2))
 
 
-def learned_vs_plain(sim, tol, cap, substeps, params, train, before=None):
+def learned_vs_plain(sim, tol, cap, substeps, params, train, before=None, warmup=0):
     import numpy as np
     snap = {k: sim.read(k) for k in ("POS", "VEL", "F", "AUX", "ID")}
+    if warmup:
+        to_residual(sim, snap, None, tol, cap, warmup, before)
+        to_residual(sim, snap, params, tol, cap, warmup, before)
     pit, pms, ppos = to_residual(sim, snap, None, tol, cap, substeps, before)
     lit, lms, lpos = to_residual(sim, snap, params, tol, cap, substeps, before)
     pi, li = float(np.mean(pit)), float(np.mean(lit))
@@ -112,6 +120,33 @@ def cpu_baseline(sim, scene, iterations, threads):
     return dict(value=sim.N / dt, unit="particle-substeps/s", cores=threads, kind="port",
                 sample=f"1 substep ({iterations} Jacobi iterations) of the settled {sim.N}-particle column, "
                        f"f32 C++ oracle, OpenMP {threads} threads, {dt:.1f} s")
+
+
+def proto2d_c1(with_cpu):
+    """Config 1: the 2D fp64 column of the prototype (40x80 at r = 0.01 m), its 300
+    substeps in one launch of the 2D engine; beside it the numpy oracle's substep."""
+    from xpbi import proto2d
+    p, x = proto2d.c1_params(), proto2d.c1_positions()
+    warm = proto2d.Prototype2D(x, p)
+    warm.step(5)
+    warm.close()
+    sim = proto2d.Prototype2D(x, p)
+    ms = sim.time(300)
+    N = sim.N
+    sim.close()
+    out = dict(workload="2D DP column 40x80 r=0.01 (3200 particles), 20 iterations, dt 1/3000, fp64",
+               substeps=300, ms_per_substep=round(ms / 300, 4), particle_substeps_per_s=round(N * 300 / (ms / 1e3), 1),
+               launch="one workgroup, all 300 substeps in one launch", cpu_baseline=None)
+    if with_cpu:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import proto2d as o2
+        prm, st = o2.c1_params(), o2.c1_state()
+        t0 = time.perf_counter()
+        o2.run(prm, st, 20)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = dict(value=round(N * 20 / dt, 1), unit="particle-substeps/s", cores=1, kind="port",
+                                   sample=f"the first 20 of the 300 substeps, numpy float64 oracle, {dt:.1f} s")
+    return out
 
 
 def mixed_scene(settle, steps):
@@ -212,12 +247,11 @@ def ensemble(I, settle, steps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--iterations", type=int, default=20)
     ap.add_argument("--settle", type=int, default=300)
     ap.add_argument("--residual-cap", type=int, default=200)
-    ap.add_argument("--residual-substeps", type=int, default=20)
     ap.add_argument("--weights", default=None, help="warm-start network (.npy) instead of training in the run")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-residual", action="store_true")
@@ -277,11 +311,16 @@ def main():
     gather = dict(achieved_GBs=round(g_bytes / (g_ms / 1e3) / 1e9, 1),
                   frac=round(g_bytes / (g_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4))
 
+    fixed = dict(value=round(value, 1), ms_per_step=round(ms_per_step, 4), steps=args.steps, iterations=I,
+                 timing="HIP-graph replay of two captured substeps, autotuned launch shape", launch_blocks=blocks)
     resid, net, train = None, None, None
     if not args.no_residual:
         print("[bench] Jacobi to residual 1e-4: plain vs learned warm start", file=sys.stderr, flush=True)
         net, train = trained_network(args.weights, log=lambda m: print("[bench] " + m, file=sys.stderr, flush=True))
-        resid = learned_vs_plain(sim, 1e-4, args.residual_cap, args.residual_substeps, net, train)
+        print("[bench] timed region: plain, then learned, from one snapshot", file=sys.stderr, flush=True)
+        resid = learned_vs_plain(sim, 1e-4, args.residual_cap, args.steps, net, train, warmup=args.warmup)
+        ms_per_step = resid["learned_ms_per_substep"]
+        value = N / (ms_per_step / 1e3)
 
     copy_bytes = 1 << 30
     copy_ms = sim.copy_bandwidth(copy_bytes, reps=5)
@@ -298,8 +337,10 @@ def main():
     if not args.no_ensemble:
         print("[bench] 64-scene batched ensemble", file=sys.stderr, flush=True)
         ens = ensemble(I, 60, 20)
-    mixed = poke = None
+    mixed = poke = c1 = None
     if not args.no_configs:
+        print("[bench] 2D column (config 1) on the fp64 engine", file=sys.stderr, flush=True)
+        c1 = proto2d_c1(not args.no_cpu)
         print("[bench] mixed water + sand scene", file=sys.stderr, flush=True)
         mixed = mixed_scene(60, 10)
         if net is not None:
@@ -311,16 +352,24 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"DP sand column 80x160x80 r=0.005 ({N} particles), plain Jacobi"
-                   if args.config == "headline" else f"DP sand column 40x80x40 r=0.01 ({N} particles)",
-                   "particles": N, "iterations": I, "dt": scene["dt"], "settle_substeps": args.settle,
-                   "timing": "HIP-graph replay of two captured substeps, autotuned launch shape",
+        "config": {"workload": (f"DP sand column 80x160x80 r=0.005 ({N} particles)" if args.config == "headline"
+                                else f"DP sand column 40x80x40 r=0.01 ({N} particles)")
+                   + (", Jacobi to residual 1e-4 from the learned warm start" if resid else
+                      f", {I} Jacobi iterations (no residual target: --no-residual)"),
+                   "particles": N, "dt": scene["dt"], "settle_substeps": args.settle,
+                   "residual": 1e-4 if resid else None, "iteration_cap": args.residual_cap if resid else I,
+                   "timing": ("eager substeps, iterations stop on the device-side residual test (polled every "
+                              "4 iterations); plain and learned from the same snapshot" if resid else
+                              "HIP-graph replay of two captured substeps, autotuned launch shape"),
                    "launch_blocks": blocks},
+        "plain_value": resid["plain_particle_substeps_per_s"] if resid else None,
+        "learned_vs_plain_time_ratio": resid["time_ratio"] if resid else None,
+        "fixed_iterations": fixed,
         "eager_ms_per_step": round(total_ms / detail_steps, 4),
         "roofline": roofline, "gather_kernels_roofline": gather, "kernels": kern,
         "stream_copy_GBs": round(copy_GBs, 1),
         "jacobi_to_residual": resid, "batched_ensemble": ens, "mixed_water_sand": mixed,
-        "vm_block_poke_to_residual": poke, "cpu_baseline": cpu,
+        "vm_block_poke_to_residual": poke, "config1_2d": c1, "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 

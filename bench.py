@@ -257,12 +257,16 @@ def main():
     ap.add_argument("--no-residual", action="store_true")
     ap.add_argument("--config", default="headline", choices=["headline", "c2"])
     ap.add_argument("--no-ensemble", action="store_true")
-    ap.add_argument("--no-configs", action="store_true", help="skip the mixed (c3) and poke (c5) scenes")
+    ap.add_argument("--no-configs", action="store_true", help="skip the mixed (c3), poke (c5) and 2D (c1) scenes")
+    ap.add_argument("--only-config1", action="store_true", help="run only the 2D column (config 1); print its object")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the substep engine has no multi-GPU path in this build "
               "(each GPU box has one MI355X); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.only_config1:
+        print(json.dumps(proto2d_c1(not args.no_cpu)))
+        return
 
     import xpbi
     from xpbi import scenes

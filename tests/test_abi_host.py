@@ -123,5 +123,7 @@ def test_napi_addon_loads():
     out = subprocess.check_output(["node", os.path.join(JS, "run.js"), "exports"], text=True)
     names = json.loads(out)
     for n in ("create", "destroy", "writeMaterials", "writeParams", "step", "read", "write", "profileSubstep",
-              "timeSubsteps", "gridInfo", "sync"):
+              "timeSubsteps", "gridInfo", "sync", "createBatch", "sceneInfo", "writeSceneMaterials",
+              "writeSceneParams", "setTolerance", "lastIterations", "setWarmstart", "graphStep", "setLaunch",
+              "getLaunch", "autotune"):
         assert n in names

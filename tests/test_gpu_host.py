@@ -46,6 +46,41 @@ def test_js_frames_match_python(tmp_path, name):
     sim.close()
 
 
+@pytest.mark.skipif(shutil.which("node") is None, reason="node not installed")
+def test_js_batch_matches_python(tmp_path):
+    """A batch of three scenes through the JavaScript host's batch constructor,
+    autotuner, learned warm start, residual loop and graph replay lands on the
+    state the Python host reaches with the same calls."""
+    names = ["Sand: two blocks collide", "Goo: Herschel-Bulkley", "Dam breach: water + sand"]
+    out = tmp_path / "batch.bin"
+    print("js batch", names, flush=True)
+    subprocess.run(["node", os.path.join(REPO, "xpbi-webgpu_amd", "js", "run.js"), "batch", ",".join(names), "2", "7",
+                    str(out)], check=True, timeout=180)
+    sl = [scenes.SCENES[n](seed=2) for n in names]
+    sim = xpbi.Simulation(sl, device=0)
+    for q, sc in enumerate(sl):
+        sim.writeParams(xpbi.params_from_scene(sc), scene=q)
+    I = max(sc["iterations"] for sc in sl)
+    water = any(sc["hasWater"] for sc in sl)
+    sim.autotune(I, water, substeps=2)
+    sim.set_launch(0)
+    net = np.zeros(5122, np.float32)
+    net[5120], net[5121] = 0.8, -1.0
+    sim.set_warmstart(net)
+    sim.set_tolerance(1e-3, 4)
+    sim.graph_step(6, 60, water)
+    sim.step(1, 60, water)
+    it, res = sim.last_iterations()
+    raw = out.read_bytes()
+    N = sim.N
+    assert len(raw) == 16 * N + 4 * N + 16
+    assert np.array_equal(np.frombuffer(raw[16 * N:20 * N], np.uint32), sim.read("ID"))
+    assert np.array_equal(np.frombuffer(raw[:16 * N], np.float32).reshape(N, 4), sim.read("POS"))
+    tail = np.frombuffer(raw[20 * N:], np.float64)
+    assert tail[0] == it and tail[1] == res and 0 < it < 60
+    sim.close()
+
+
 def test_render_view_and_dump(tmp_path):
     """The exported device pointers hold what xpbi_read returns; MATS holds the
     written records; the PLY dump carries the read-back state."""

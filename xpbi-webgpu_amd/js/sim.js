@@ -51,56 +51,74 @@ function requestDevice(opts) {
   };
 }
 
+// grid of one scene: sized to its domain with a 2-cell pad (sim.js:20-26)
+function sceneGrid(scene) {
+  const cs = 2 * scene.h;
+  const pad = 2;
+  const gridOrigin = scene.domainMin.map(v => v - pad * cs);
+  const gridDims = [0, 1, 2].map(i => Math.ceil((scene.domainMax[i] - scene.domainMin[i]) / cs) + 2 * pad);
+  return { h: scene.h, support: cs, V0: scene.h * scene.h * scene.h, gridOrigin, gridDims,
+           numCells: gridDims[0] * gridDims[1] * gridDims[2], N: scene.positions.length / 3 };
+}
+
 class Simulation {
+  // scene: one scene (the reference's constructor, sim.js:10-30) or an array of
+  // scenes, run as one batch: one launch per kernel for all of them, each with
+  // its own grid, materials and Params (writeMaterials / writeParams take the
+  // scene's index)
   constructor(device, scene) {
     this.device = device;
-    this.scene = scene;
-    const N = scene.positions.length / 3;
-    this.N = N;
-    this.h = scene.h;
-    this.support = 2 * scene.h;
-    this.V0 = scene.h * scene.h * scene.h;
+    this.scenes = Array.isArray(scene) ? scene : [scene];
+    this.scene = this.scenes[0];
+    this.grids = this.scenes.map(sceneGrid);
+    const g = this.grids[0];
+    this.N = this.grids.reduce((a, q) => a + q.N, 0);
+    this.h = g.h;
+    this.support = g.support;
+    this.V0 = g.V0;
     this.frame = 0;
-    // grid sized to domain with a 2-cell pad (sim.js:20-26)
-    const cs = this.support;
-    const pad = 2;
-    this.gridOrigin = scene.domainMin.map(v => v - pad * cs);
-    this.gridDims = [0, 1, 2].map(i => Math.ceil((scene.domainMax[i] - scene.domainMin[i]) / cs) + 2 * pad);
-    this.numCells = this.gridDims[0] * this.gridDims[1] * this.gridDims[2];
+    this.gridOrigin = g.gridOrigin;
+    this.gridDims = g.gridDims;
+    this.numCells = g.numCells;
     this.createBuffers();
     this.createPipelines();
   }
 
   // sim.js:32-82: the library allocates every device buffer and uploads the state
   createBuffers() {
-    const s = this.scene;
-    const N = this.N;
     const f32 = a => (a instanceof Float32Array ? a : Float32Array.from(a));
-    let tags = s.tags ? f32(s.tags) : null;
-    if (!tags) {
-      tags = new Float32Array(N);
-      for (let i = 0; i < N; i++) tags[i] = s.matIds[i];
-    }
-    this.handle = loadAddon().create({
-      N,
-      positions: f32(s.positions),
-      tags,
-      states: s.states ? f32(s.states) : undefined,
-      velocities: s.velocities ? f32(s.velocities) : undefined,
-      h: s.h,
-      domainMin: s.domainMin,
-      domainMax: s.domainMax,
-      device: this.device ? this.device.ordinal : 0,
+    const descs = this.scenes.map((s, q) => {
+      const N = this.grids[q].N;
+      let tags = s.tags ? f32(s.tags) : null;
+      if (!tags) {
+        tags = new Float32Array(N);
+        for (let i = 0; i < N; i++) tags[i] = s.matIds[i];
+      }
+      return {
+        N, positions: f32(s.positions), tags,
+        states: s.states ? f32(s.states) : undefined,
+        velocities: s.velocities ? f32(s.velocities) : undefined,
+        h: s.h, domainMin: s.domainMin, domainMax: s.domainMax,
+        device: this.device ? this.device.ordinal : 0,
+      };
     });
-    const g = loadAddon().gridInfo(this.handle);
-    if (g.numCells !== this.numCells || g.gridDims.some((d, i) => d !== this.gridDims[i])) {
-      throw new Error('grid mismatch between the host and libxpbi');
-    }
-    this.writeMaterials(s.materials);
+    this.handle = descs.length === 1 ? loadAddon().create(descs[0]) : loadAddon().createBatch(descs);
+    this.grids.forEach((g, q) => {
+      const info = loadAddon().sceneInfo(this.handle, q);
+      if (info.numCells !== g.numCells || info.gridDims.some((d, i) => d !== g.gridDims[i])) {
+        throw new Error('grid mismatch between the host and libxpbi');
+      }
+      g.firstParticle = info.firstParticle;
+      g.firstCell = info.firstCell;
+    });
+    this.totalCells = this.grids.reduce((a, q) => a + q.numCells, 0);
+    this.scenes.forEach((s, q) => this.writeMaterials(s.materials, q));
   }
 
-  // sim.js:84-99
-  writeMaterials(materials) {
+  // sim.js:84-99 (scene: index in a batch)
+  writeMaterials(materials, scene) {
+    const q = scene || 0;
+    const V0 = this.grids[q].V0;
     const arr = new Float32Array(8 * MAT_STRIDE);
     const u32 = new Uint32Array(arr.buffer);
     materials.forEach((m, i) => {
@@ -113,32 +131,34 @@ class Simulation {
       const c = m.color || [0, 0, 0];
       arr[o + 8] = c[0]; arr[o + 9] = c[1]; arr[o + 10] = c[2]; arr[o + 11] = 1;
       arr[o + 12] = def(m.boundFriction, 0.4);
-      arr[o + 13] = m.density * this.V0;
+      arr[o + 13] = m.density * V0;
     });
-    loadAddon().writeMaterials(this.handle, new Uint8Array(arr.buffer, 0, 64 * Math.min(8, materials.length)));
+    loadAddon().writeSceneMaterials(this.handle, q, new Uint8Array(arr.buffer, 0, 64 * Math.min(8, materials.length)));
   }
 
   // the kernels are compiled once into libxpbi.so (sim.js:101-156 built WGSL pipelines)
   createPipelines() {}
 
-  // sim.js:159-186: the 256-B Params record
-  writeParams(p) {
+  // sim.js:159-186: the 256-B Params record (scene: index in a batch)
+  writeParams(p, scene) {
+    const q = scene || 0;
+    const g = this.grids[q];
     const buf = new ArrayBuffer(256);
     const f = new Float32Array(buf);
     const i32 = new Int32Array(buf);
     const u32 = new Uint32Array(buf);
-    f.set(this.gridOrigin, 0); f[3] = this.support;
-    i32[4] = this.gridDims[0]; i32[5] = this.gridDims[1]; i32[6] = this.gridDims[2];
-    u32[7] = this.numCells;
+    f.set(g.gridOrigin, 0); f[3] = g.support;
+    i32[4] = g.gridDims[0]; i32[5] = g.gridDims[1]; i32[6] = g.gridDims[2];
+    u32[7] = g.numCells;
     f.set(p.domainMin, 8); f[11] = p.dt;
-    f.set(p.domainMax, 12); f[15] = this.h;
-    f.set(p.gravity, 16); f[19] = this.support;
+    f.set(p.domainMax, 12); f[15] = g.h;
+    f.set(p.gravity, 16); f[19] = g.support;
     f.set(p.colliderPos || [0, 0, 0], 20); f[23] = def(p.colliderR, 0);
     f.set(p.colliderVel || [0, 0, 0], 24); f[27] = p.colliderActive ? 1 : 0;
-    u32[28] = this.N; f[29] = this.V0; f[30] = p.xsphC; f[31] = p.omega;
+    u32[28] = g.N; f[29] = g.V0; f[30] = p.xsphC; f[31] = p.omega;
     f[32] = p.pbfEps; f[33] = p.sCorrK; f[34] = def(p.time, 0); f[35] = p.contactOmega;
     f[36] = def(p.kDamp, 0.4);
-    loadAddon().writeParams(this.handle, buf);
+    loadAddon().writeSceneParams(this.handle, q, buf);
   }
 
   initProfiler() { return true; }
@@ -162,6 +182,21 @@ class Simulation {
     return loadAddon().timeSubsteps(this.handle, substeps, iterations, !!hasWater);
   }
 
+  // ---- beyond the reference: residual-driven Jacobi, learned warm start,
+  // HIP-graph replay and launch shapes (include/xpbi.h)
+  // iterations become a cap: stop at residual tol, tested every checkEvery iterations
+  setTolerance(tol, checkEvery) { loadAddon().setTolerance(this.handle, tol, checkEvery === undefined ? 4 : checkEvery); }
+  // {iterations, residual} of the last substep
+  lastIterations() { return loadAddon().lastIterations(this.handle); }
+  // Float32Array of the network's 5122 parameters, or null for plain Jacobi
+  setWarmstart(params) { loadAddon().setWarmstart(this.handle, params || null); }
+  // substeps replayed from a captured HIP graph (same results as step)
+  graphStep(substeps, iterations, hasWater) { loadAddon().graphStep(this.handle, substeps, iterations, !!hasWater); }
+  setLaunch(blocks) { loadAddon().setLaunch(this.handle, blocks); }
+  getLaunch() { return loadAddon().getLaunch(this.handle); }
+  // times the launch shapes on the live state, keeps the fastest: {blocks, ms}
+  autotune(iterations, hasWater, substeps) { return loadAddon().autotune(this.handle, iterations, !!hasWater, substeps || 2); }
+
   read(field) {
     const ab = loadAddon().read(this.handle, FIELD[field]);
     return (field === 'PERM' || field === 'ID' || field === 'CELLSTART') ? new Uint32Array(ab) : new Float32Array(ab);
@@ -179,4 +214,4 @@ class Simulation {
   }
 }
 
-module.exports = { Simulation, MAT, FIELD, requestDevice, loadAddon };
+module.exports = { Simulation, MAT, FIELD, requestDevice, loadAddon, sceneGrid };

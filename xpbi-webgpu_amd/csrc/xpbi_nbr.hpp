@@ -106,7 +106,10 @@ __device__ __forceinline__ void forSupportG(const KP& P, const u32* __restrict__
     // so a batch's tail gathers i itself); the base is added when the batch is
     // used, so that no arithmetic on a loaded entry sits inside the load's
     // predicate (which would wait on every load in flight).  Slot kk's row
-    // nl.lst + kk N is uniform: the entry load is a scalar base plus 2 i.
+    // nl.lst + kk N is uniform: the entry load is a scalar base plus 2 i.  (The
+    // predicate stays: unpredicated entry loads, over a list padded with the own
+    // offset, made the compiler wait for a whole batch of gathers before its
+    // first use and measured 13 % slower.)
     const u32 self = i - hd.x;
     u32 en[kB];
 #pragma unroll
@@ -154,8 +157,11 @@ struct MatLds {
   DevMat m[8];
   float mass[8];
   float invMass[8];
-  u32 solidMask;  // bit k set when material k is not water
+  float wc[8][8];  // contact weight m_b / (m_a + m_b) of material pair (a, b) (simkernels.js:346)
+  u32 solidMask;   // bit k set when material k is not water
 };
+// every material of the scene is solid: the per-neighbour phase tests vanish
+__device__ __forceinline__ bool allSolid(const MatLds& t) { return t.solidMask == 0xFFu; }
 __device__ __forceinline__ void loadMatLds(MatLds& t, const DevMat* __restrict__ mats) {
   const u32 tid = threadIdx.x;
   __syncthreads();  // a previous scene's table may still be read
@@ -163,6 +169,10 @@ __device__ __forceinline__ void loadMatLds(MatLds& t, const DevMat* __restrict__
   if (tid < 8) {
     t.mass[tid] = mats[tid].mass;
     t.invMass[tid] = 1.0f / mats[tid].mass;
+  }
+  if (tid < 64) {
+    const float ma = mats[tid >> 3].mass, mb = mats[tid & 7u].mass;
+    t.wc[tid >> 3][tid & 7u] = mb / (ma + mb);
   }
   if (tid == 0) {
     u32 mask = 0;
@@ -215,7 +225,8 @@ __device__ __forceinline__ void momentsAdd(Moments& m, f3 g, f3 w, float Vb, flo
     m.q12 += cg.y * g.z;
   }
 }
-template <bool kWithQ>
+// kAllSolid: the scene has no water, every neighbour contributes
+template <bool kWithQ, bool kAllSolid>
 __device__ __forceinline__ Moments neighbourMoments(const KP& P, const u32* __restrict__ cs,
                                                     const float4* __restrict__ pos, const float4* __restrict__ vin,
                                                     const NLc& nl, const float* invMass, u32 solidMask, u32 i,
@@ -225,7 +236,7 @@ __device__ __forceinline__ Moments neighbourMoments(const KP& P, const u32* __re
   forSupportG(P, cs, nl, i, xi, [&](u32 b) { return PV{gat(pos, b), gat(vin, b)}; },
               [&](u32 b, const PV& r, f3 dxv, float r2) {
                 const float4 pb = r.p, vb = r.v;
-                if (solidTag(solidMask, pb.w)) {
+                if (kAllSolid || solidTag(solidMask, pb.w)) {
                   momentsAdd<kWithQ>(m, gradWfast(P, dxv, r2), (xyz(vb) - vi) * vb.w, vb.w,
                                      invMass[matOfTag(pb.w)]);
                 }
@@ -378,9 +389,10 @@ struct SpGather {
 
 // simkernels.js:338-355 for one neighbour: constraint gather dv += V_i S_b gradW(x_b - x_i)
 // and the 0.75h contact on predicted positions
-__device__ __forceinline__ void solveBPair(const KP& P, const MatLds& T, float Vi, float mass_i, f3 xci, f3 dxv,
+template <bool kAllSolid>
+__device__ __forceinline__ void solveBPair(const KP& P, const MatLds& T, float Vi, const float* wc_i, f3 xci, f3 dxv,
                                            float r2, float4 pb4, float4 vb, const f33& Spb, f3* dv, f3* corr) {
-  if (solidTag(T.solidMask, pb4.w)) {
+  if (kAllSolid || solidTag(T.solidMask, pb4.w)) {
     const f3 dxp = -dxv;  // x_b - x_i
     *dv = *dv + Vi * mul(Spb, gradWfast(P, dxp, r2));
     const f3 xcb = xyz(pb4) + P.dt * xyz(vb);
@@ -390,13 +402,13 @@ __device__ __forceinline__ void solveBPair(const KP& P, const MatLds& T, float V
     if (d2 < minD * minD && d2 > 1e-18f) {
       float idist = xrsq(d2);
       float dist = d2 * idist;
-      float mb = T.mass[matOfTag(pb4.w)];
-      float w = mb / (mass_i + mb);
+      float w = wc_i[matOfTag(pb4.w)];
       *corr = *corr + dxc * (idist * ((minD - dist) * w));
     }
   }
 }
 
+template <bool kAllSolid>
 __device__ __forceinline__ void solveBGatherGlobal(const KP& P, const MatLds& T, u32 i, float4 pi, float4 vi4,
                                                    const float4* __restrict__ pos, const float4* __restrict__ vin,
                                                    M3C Sp, const float4* __restrict__ selfDv,
@@ -405,14 +417,14 @@ __device__ __forceinline__ void solveBGatherGlobal(const KP& P, const MatLds& T,
   const float Vi = vi4.w;
   const f3 xi = xyz(pi);
   const f3 xci = xi + P.dt * xyz(vi4);
-  const float mass_i = T.mass[matOfTag(pi.w)];
+  const float* wc_i = T.wc[matOfTag(pi.w)];
   // indexed by pointer, not by gat(): the compiler then issues a batch's loads
   // grouped by array, which measured 30% faster for these five arrays than the
   // per-neighbour interleaving it emits for 32-bit offsets
   forSupportG<kBatchB>(P, cs, nl, i, xi, [&](u32 b) { return SpGather{pos[b], vin[b], Sp.a[b], Sp.b[b], Sp.c[b]}; },
               [&](u32 b, const SpGather& g, f3 dxv, float r2) {
                 const f33 Spb{{f3{g.a.x, g.a.y, g.a.z}, f3{g.a.w, g.b.x, g.b.y}, f3{g.b.z, g.b.w, g.c}}};
-                solveBPair(P, T, Vi, mass_i, xci, dxv, r2, g.p, g.v, Spb, dv, corr);
+                solveBPair<kAllSolid>(P, T, Vi, wc_i, xci, dxv, r2, g.p, g.v, Spb, dv, corr);
               });
 }
 

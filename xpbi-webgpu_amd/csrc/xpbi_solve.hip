@@ -10,6 +10,8 @@
 // and stored as a slot-major list; every later sweep walks that list instead of
 // re-testing the ~216 candidates of the 27-cell stencil.
 #include "xpbi_mlp.hpp"
+#include <type_traits>
+
 #include "xpbi_nbr.hpp"
 
 namespace xpbi {
@@ -88,6 +90,8 @@ __global__ __launch_bounds__(256) void k_computeL(const KP* __restrict__ kps, Wo
         }
       }
     }
+    // the particle's own offset must fit too (the sweeps' batch tails gather i)
+    flag |= (i - base) > kNbrOffMax ? kNbrFlag : 0u;
     const u32 ns = min(n, kStage);
     for (u32 k = 0; k < ns; k++) nLst[(size_t)k * NT + i] = stage[k][tid];
     nHead[i] = make_uint2(base, n | flag);
@@ -124,7 +128,9 @@ __global__ __launch_bounds__(256) void k_solveA(const KP* __restrict__ kps, Work
     Moments mo;
     const u32 mi = matOfTag(pi.w);
     const bool solid = (T.solidMask >> mi) & 1u;
-    if (solid) mo = neighbourMoments<true>(P, cs, pos, vin, nl, T.invMass, T.solidMask, i, xyz(pi), vi);
+    if (solid)
+      mo = allSolid(T) ? neighbourMoments<true, true>(P, cs, pos, vin, nl, T.invMass, T.solidMask, i, xyz(pi), vi)
+                       : neighbourMoments<true, false>(P, cs, pos, vin, nl, T.invMass, T.solidMask, i, xyz(pi), vi);
     const float hint = (hbHint && T.m[mi].kind == MAT_HB) ? hbHint[i] : -1.0f;
     solveAFinish(P, T, i, pi, vi, solid, mo, Fm, Lm, aux, Sp, selfDv, &r2, &c2, hint);
   });
@@ -198,8 +204,10 @@ __global__ __launch_bounds__(256) void k_solveB(const KP* __restrict__ kps, Work
     const float4 vi4 = vin[i];
     const u32 mi = matOfTag(pi.w);
     f3 dv = splat3(0.0f), corr = splat3(0.0f);
-    if ((T.solidMask >> mi) & 1u)
-      solveBGatherGlobal(P, T, i, pi, vi4, pos, vin, Sp, selfDv, cs, nl, &dv, &corr);
+    if ((T.solidMask >> mi) & 1u) {
+      if (allSolid(T)) solveBGatherGlobal<true>(P, T, i, pi, vi4, pos, vin, Sp, selfDv, cs, nl, &dv, &corr);
+      else solveBGatherGlobal<false>(P, T, i, pi, vi4, pos, vin, Sp, selfDv, cs, nl, &dv, &corr);
+    }
     solveBFinish(P, T, i, pi, vi4, dv, corr, vout, xcOut);
   });
 }
@@ -315,10 +323,17 @@ __global__ __launch_bounds__(256) void k_xsph(const KP* __restrict__ kps, WorkTa
     const f3 vi = xyz(vi4);
     const bool iSolid = solidTag(T.solidMask, pi.w);
     f3 acc = splat3(0.0f);
-    forSupportG(P, cs, nl, i, xi, [&](u32 b) { return PV{gat(pos, b), gat(vin, b)}; },
-                [&](u32 b, const PV& r, f3 dxv, float r2) {
-                  if (solidTag(T.solidMask, r.p.w) == iSolid) acc = acc + r.v.w * (xyz(r.v) - vi) * kernelWfast(P, r2);
-                });
+    // kAll: no water in the scene, every neighbour shares the particle's phase
+    auto gather = [&](auto all) {
+      constexpr bool kAll = decltype(all)::value;
+      forSupportG(P, cs, nl, i, xi, [&](u32 b) { return PV{gat(pos, b), gat(vin, b)}; },
+                  [&](u32 b, const PV& r, f3 dxv, float r2) {
+                    if (kAll || solidTag(T.solidMask, r.p.w) == iSolid)
+                      acc = acc + r.v.w * (xyz(r.v) - vi) * kernelWfast(P, r2);
+                  });
+    };
+    if (allSolid(T)) gather(std::true_type{});
+    else gather(std::false_type{});
     float c = P.xsphC;
     if (!iSolid) c = P.xsphC * 10.0f;
     f3 v = vi + c * acc;
@@ -347,7 +362,9 @@ __global__ __launch_bounds__(256) void k_updateF(const KP* __restrict__ kps, Wor
     MatV m = hardenedMat(matv(T, mi), state);
     const f3 xi = xyz(pi);
     const f33 Li = ldm(Lm, i);
-    const Moments mo = neighbourMoments<false>(P, cs, pos, vin, nl, T.invMass, T.solidMask, i, xi, xyz(vin[i]));
+    const f3 vi = xyz(vin[i]);
+    const Moments mo = allSolid(T) ? neighbourMoments<false, true>(P, cs, pos, vin, nl, T.invMass, T.solidMask, i, xi, vi)
+                                   : neighbourMoments<false, false>(P, cs, pos, vin, nl, T.invMass, T.solidMask, i, xi, vi);
     const f33 gradV = mul(mo.M, transpose(Li));
     f33 Ftr = mul(madd(mident<float>(), mscale(gradV, P.dt)), ldm(Fm, i));
     SVD3<float> sv = svd3_guarded(Ftr);
@@ -417,7 +434,9 @@ __global__ __launch_bounds__(256) void k_features(const KP* __restrict__ kps, Wo
     bool wrote = false;
     if ((T.solidMask >> mi) & 1u) {
       const f3 vi = xyz(vin[i]);
-      const Moments mo = neighbourMoments<true>(P, cs, pos, vin, nl, T.invMass, T.solidMask, i, xyz(pi), vi);
+      const Moments mo =
+          allSolid(T) ? neighbourMoments<true, true>(P, cs, pos, vin, nl, T.invMass, T.solidMask, i, xyz(pi), vi)
+                      : neighbourMoments<true, false>(P, cs, pos, vin, nl, T.invMass, T.solidMask, i, xyz(pi), vi);
       const float4 a = aux[i];
       const Constraint c = evalConstraint(P, T, i, pi, a, mo, Fm, Lm, -1.0f, true);
       if (c.active) {

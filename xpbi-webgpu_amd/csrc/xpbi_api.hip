@@ -50,6 +50,8 @@ void free_all(Sim* s) {
   s->allocs.clear();
   if (s->ctlHost) hipHostFree(s->ctlHost);
   s->ctlHost = nullptr;
+  if (s->mirror) hipHostFree(s->mirror);
+  s->mirror = nullptr;
   for (hipEvent_t e : s->events) hipEventDestroy(e);
   s->events.clear();
   if (s->stream) hipStreamDestroy(s->stream);
@@ -276,6 +278,8 @@ int xpbi_create_batch(const xpbi_scene_desc* descs, int count, xpbi_sim** out) {
   A(dalloc(s, &s->residOut, 2ull * kMaxResidIters));
   A(dalloc(s, &s->ctl, 1));
   A(hipHostMalloc((void**)&s->ctlHost, sizeof(SolveCtl), hipHostMallocDefault));
+  A(hipHostMalloc((void**)&s->mirror, sizeof(CtlMirror), hipHostMallocCoherent | hipHostMallocMapped));
+  if (s->mirror) std::memset(s->mirror, 0, sizeof(CtlMirror));
   if (!ok) {
     free_all(s);
     delete h;

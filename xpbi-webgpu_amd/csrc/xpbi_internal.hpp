@@ -43,6 +43,16 @@ struct SolveCtl {
   double r2;  // last residual numerator^2
 };
 
+// what the device reports to the host after each residual test, written by the
+// device straight into pinned host memory, so that the host polls it while the
+// stream keeps running (no copy, no synchronisation)
+struct CtlMirror {
+  u32 gen;    // solve the fields below belong to (written last)
+  u32 iters;  // iterations whose residual test has run
+  u32 done;   // converged
+  u32 pad;
+};
+
 // iteration k of the loop is skipped once an earlier iteration was the last:
 // the stopping iteration itself (iters == k + 1) still runs its solveB / PBF
 __device__ __forceinline__ bool skipIteration(const SolveCtl* ctl, u32 k) {
@@ -118,7 +128,9 @@ struct Sim {
   float tol = 0.0f;
   int checkEvery = 4;       // host polls the done flag every checkEvery iterations (0: never)
   SolveCtl* ctl = nullptr;  // device
-  SolveCtl* ctlHost = nullptr;  // pinned mirror for polling
+  SolveCtl* ctlHost = nullptr;  // pinned copy of ctl
+  CtlMirror* mirror = nullptr;  // pinned, written by k_resid_final
+  u32 solveGen = 0;             // generation of the current residual loop
   int lastIters = 0;        // fixed-iteration count of the last substep (tol == 0)
   int warmIters = 0;        // 1 when the last substep's first iteration was the learned one
 

@@ -10,6 +10,8 @@
 // and stored as a slot-major list; every later sweep walks that list instead of
 // re-testing the ~216 candidates of the 27-cell stencil.
 #include "xpbi_mlp.hpp"
+#include <atomic>
+#include <chrono>
 #include <type_traits>
 
 #include "xpbi_nbr.hpp"
@@ -140,7 +142,8 @@ __global__ __launch_bounds__(256) void k_solveA(const KP* __restrict__ kps, Work
 // deterministic second stage of the residual reduction; with ctl, also the
 // stopping test of iteration k (skipped once the loop is done)
 __global__ __launch_bounds__(256) void k_resid_final(const double* __restrict__ part, u32 nb, double* __restrict__ out,
-                                                     SolveCtl* __restrict__ ctl, u32 k, double tol2) {
+                                                     SolveCtl* __restrict__ ctl, u32 k, double tol2,
+                                                     CtlMirror* __restrict__ mirror, u32 gen) {
   if (skipIteration(ctl, k)) return;
   __shared__ double s_out[2];
   double a = 0.0, b = 0.0;
@@ -161,6 +164,13 @@ __global__ __launch_bounds__(256) void k_resid_final(const double* __restrict__ 
       ctl->iters = k + 1;
       ctl->r2 = s_out[0];
       if (c0 <= 0.0 || s_out[0] <= tol2 * c0) ctl->done = 1u;
+      if (mirror) {
+        volatile CtlMirror* m = mirror;
+        m->iters = k + 1;
+        m->done = ctl->done;
+        __threadfence_system();
+        m->gen = gen;
+      }
     }
   }
 }
@@ -522,6 +532,35 @@ static inline u32 cdiv(u64 a, u64 b) { return (u32)((a + b - 1) / b); }
 static inline M3C cm(const M3P& m) { return M3C{m.a, m.b, m.c}; }
 
 // sim.js:255-266 after the reorder; ends with s->vcur on the final velocity
+// the device's report for the current loop (generation gen): true once it has
+// converged; returns false when it has run the residual test of iteration
+// `need` (or need < 0) without converging.  Spins on the pinned mirror; past
+// 0.1 ms it also lets the runtime flush the stream, past 2 s it synchronises.
+static bool polledDone(Sim* s, u32 gen, int need) {
+  volatile CtlMirror* m = s->mirror;
+  const auto t0 = std::chrono::steady_clock::now();
+  bool synced = false;
+  for (u32 spin = 0;; spin++) {
+    if (m->gen == gen) {
+      std::atomic_thread_fence(std::memory_order_acquire);
+      if (m->done) return true;
+      if ((int)m->iters > need) return false;
+    } else if (need < 0) {
+      return false;
+    }
+    if (synced) return false;  // the stream is idle and the report is what it is
+    if ((spin & 255u) == 255u) {
+      const double us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+      if (us > 2e6) {
+        hipStreamSynchronize(s->stream);
+        synced = true;
+      } else if (us > 100.0) {
+        hipStreamQuery(s->stream);
+      }
+    }
+  }
+}
+
 void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
   const u32 gN = s->grid;  // blocks of a particle launch (and residual partials)
   const KP* P = s->dKP;
@@ -540,6 +579,7 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
   SolveCtl* ctl = loop ? s->ctl : nullptr;
   const double tol2 = (double)s->tol * (double)s->tol;
   if (loop) hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, st, ctl);
+  const u32 gen = loop ? ++s->solveGen : 0u;
   const float* hbHint = s->wsMode == 1 ? (const float*)s->wsHb : nullptr;
   int warm = 0;  // the learned first iteration ran
   if (s->wsMode != 0) {
@@ -574,10 +614,10 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
   int launched = 0;
   for (int k = 0; k < iterations - warm; k++) {
     if (loop && s->checkEvery > 0 && k > 0 && k % s->checkEvery == 0) {
-      // stop enqueuing once the device has flagged convergence
-      hipMemcpyAsync(s->ctlHost, ctl, sizeof(SolveCtl), hipMemcpyDeviceToHost, st);
-      hipStreamSynchronize(st);
-      if (s->ctlHost->done) break;
+      // stop enqueuing once the device has flagged convergence; stay at most two
+      // check intervals ahead of the iterations whose test the device has run
+      // (the queued ones keep it busy meanwhile; skipped ones return at entry)
+      if (polledDone(s, gen, k - 2 * s->checkEvery)) break;
     }
     double* out = (s->residOn && k < kMaxResidIters) ? s->residOut + 2 * k : nullptr;
     double* part = (out || loop) ? s->residPart : nullptr;
@@ -586,7 +626,8 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
                        (const u32*)s->cellStart, nl, part, (const SolveCtl*)ctl, (u32)k, hbHint);
     s->prof_mark("solveA");
     if (part) {
-      hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(256), 0, st, (const double*)part, gN, out, ctl, (u32)k, tol2);
+      hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(256), 0, st, (const double*)part, gN, out, ctl, (u32)k, tol2,
+                         loop ? s->mirror : (CtlMirror*)nullptr, gen);
       if (out) s->residIters = k + 1;
     }
     XPBI_LAUNCH_W(k_solveB, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,

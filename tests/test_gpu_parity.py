@@ -212,6 +212,18 @@ def test_substep_parity_sand_column(oracle):
     sim.close()
 
 
+def test_substep_parity_headline_column(oracle):
+    """The benchmark's 1,024,000-particle DP column at full size, landed (300 substeps):
+    the sort bit-exact and one substep (20 iterations) within the bar, against the f32
+    oracle, its 1-ulp perturbations and the f64 oracle (OpenMP, a few seconds each)."""
+    scene = scenes.config_headline()
+    sim = make_sim(scene)
+    print("headline column: settling 300 substeps", flush=True)
+    sim.step(300, 20, False)
+    one_substep_parity(sim, scene, oracle, "1M DP column after 300 substeps")
+    sim.close()
+
+
 @pytest.mark.parametrize("name", list(scenes.SCENES))
 def test_substep_parity_reference_scenes(oracle, name):
     """Every material model of the reference (DP, NACC, VM, HB, snow, water+sand, elastic)."""

@@ -146,7 +146,7 @@ def proto2d_c1(with_cpu):
     sim.close()
     out = dict(workload="2D DP column 40x80 r=0.01 (3200 particles), 20 iterations, dt 1/3000, fp64",
                substeps=300, ms_per_substep=round(ms / 300, 4), particle_substeps_per_s=round(N * 300 / (ms / 1e3), 1),
-               launch="one workgroup, all 300 substeps in one launch", cpu_baseline=None)
+               launch="cooperative grid, one thread per particle, all 300 substeps in one launch", cpu_baseline=None)
     if with_cpu:
         sys.path.insert(0, os.path.join(REPO, "oracle"))
         import proto2d as o2

@@ -337,8 +337,8 @@ int xpbi_mlp_train(xpbi_mlp* mlp, int epochs, int batch, float lr, uint64_t seed
  * The prototype is a script (no API): a sand column whose state is x, v, F
  * (xpbi2d.py:52-58) advanced by the main loop (xpbi2d.py:123-241) with
  * constants fixed at xpbi2d.py:29-49.  Here the constants are xpbi2d_params and
- * the loop is xpbi2d_step.  One workgroup per scene runs a call's substeps in
- * one launch, in float64. */
+ * the loop is xpbi2d_step, which runs a call's substeps in one launch, in
+ * float64. */
 typedef struct xpbi2d_params {
   double h;           /* spacing = kernel radius (xpbi2d.py:29-30); support 2h, V0 = h^2 */
   double E, nu, rho;  /* StVK-Hencky moduli and density (xpbi2d.py:32-36) */
@@ -365,6 +365,11 @@ const char* xpbi2d_last_error(xpbi2d_sim* sim);
 /* substeps iterations of xpbi2d.py:123-237; returns when they are done.
  * XPBI_ERR_STATE when a particle had more than 48 neighbours within 2h. */
 int xpbi2d_step(xpbi2d_sim* sim, int substeps);
+/* How a scene's substeps run: single = 0 (the default where the device supports
+ * cooperative launches) a grid of 256-thread workgroups, one thread per particle,
+ * separated by grid barriers; single = 1 one workgroup of 1024 threads.  Results
+ * are the same bit for bit. */
+int xpbi2d_set_workgroups(xpbi2d_sim* sim, int single);
 /* xpbi2d_step with a HIP event pair around the launch: *ms = its duration. */
 int xpbi2d_time(xpbi2d_sim* sim, int substeps, float* ms);
 /* count = the field's size in doubles. */

@@ -82,3 +82,19 @@ def test_neighbour_overflow_is_an_error():
     with pytest.raises(XpbiError, match="neighbours"):
         sim.step(1)
     sim.close()
+
+
+def test_cooperative_grid_equals_single_workgroup():
+    """The cooperative grid (one thread per particle, grid barriers) and one workgroup
+    (workgroup barriers) run the same arithmetic: identical state, bit for bit."""
+    p, x = host.c1_params(), host.c1_positions()
+    out = {}
+    for single in (False, True):
+        sim = host.Prototype2D(x, p)
+        sim.set_workgroups(single)
+        ms = sim.time(20)
+        print(f"{'single workgroup' if single else 'cooperative grid'}: {ms / 20:.3f} ms per substep", flush=True)
+        out[single] = {k: sim.read(k) for k in ("X", "V", "F", "LAMBDA")}
+        sim.close()
+    for k in out[False]:
+        assert np.array_equal(out[False][k], out[True][k]), k

@@ -1,10 +1,11 @@
 // 2D fp64 XPBI engine: the algorithm of prototype/xpbi2d.py:123-241 on the GPU.
 //
 // The prototype's scenes are small (666 particles at its own constants, 3,200
-// for the 40 x 80 column at r = 0.01), so a scene is one persistent workgroup of
-// 1024 threads that runs every substep of a call in a single launch: phases are
-// separated by workgroup barriers, the state (a few hundred KB) stays in L2, and
-// no launch or host round trip sits between the ~90 phases of a substep.
+// for the 40 x 80 column at r = 0.01), so one launch runs every substep of a call:
+// a cooperative grid of 256-thread workgroups, one thread per particle, whose ~95
+// phases per substep are separated by grid barriers (or, selectable, a single
+// workgroup of 1024 threads with workgroup barriers).  The state (a few hundred
+// KB) stays in L2 and no launch or host round trip sits between the phases.
 //
 // Arithmetic is float64 without contraction; every sum runs over a particle's
 // neighbours in ascending index order and every expression is evaluated in the
@@ -14,6 +15,7 @@
 // Neighbours: each substep re-grids x^n into cells of side 2h (counting sort
 // within the workgroup), and each particle's list holds every b != p with
 // |x_p - x_b|^2 <= (2h)^2 (xpbi2d.py:125-129), sorted ascending.
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -50,6 +52,20 @@ struct Buf {
   m2 *F, *L, *G;
   double *Vn, *lam, *dlam;
   u32 *cellCount, *cellStart, *cursor, *sorted, *cellOf, *nbr, *cnt, *err;
+};
+
+// the threads that run a scene: a cooperative grid (barrier: grid sync) or one
+// workgroup (barrier: workgroup barrier); phase loops stride by `size`
+template <bool kGrid>
+struct Team {
+  u32 rank, size;
+  __device__ Team()
+      : rank(kGrid ? blockIdx.x * blockDim.x + threadIdx.x : threadIdx.x),
+        size(kGrid ? gridDim.x * blockDim.x : blockDim.x) {}
+  __device__ void sync() const {
+    if (kGrid) cooperative_groups::this_grid().sync();
+    else __syncthreads();
+  }
 };
 
 // ---------------------------------------------------------------- math
@@ -158,12 +174,13 @@ __device__ __forceinline__ double stvk(const Prm& P, double s0, double s1, doubl
   for (u32 k_ = 0, n_ = B.cnt[p]; k_ < n_; k_++)          \
     if (const u32 b = B.nbr[(size_t)k_ * P.N + (p)]; true)
 
-__device__ void buildNeighbours(const Prm& P, const Buf& B) {
-  const u32 tid = threadIdx.x, N = P.N, C = P.C;
+template <bool kGrid>
+__device__ void buildNeighbours(const Prm& P, const Buf& B, const Team<kGrid>& T) {
+  const u32 N = P.N, C = P.C;
   __shared__ u32 s_sum[kThreads];
-  for (u32 c = tid; c < C; c += kThreads) B.cellCount[c] = 0u;
-  __syncthreads();
-  for (u32 p = tid; p < N; p += kThreads) {
+  for (u32 c = T.rank; c < C; c += T.size) B.cellCount[c] = 0u;
+  T.sync();
+  for (u32 p = T.rank; p < N; p += T.size) {
     const d2 x = B.x[p];
     int cx = (int)floor(x.x / P.support), cy = (int)floor(x.y / P.support);
     cx = min(max(cx, 0), P.ncx - 1);
@@ -172,31 +189,35 @@ __device__ void buildNeighbours(const Prm& P, const Buf& B) {
     B.cellOf[p] = c;
     atomicAdd(&B.cellCount[c], 1u);
   }
-  __syncthreads();
-  // exclusive scan: a contiguous chunk of cells per thread, then the chunk sums
-  const u32 chunk = (C + kThreads - 1) / kThreads;
-  const u32 c0 = min(tid * chunk, C), c1 = min(c0 + chunk, C);
-  u32 s = 0;
-  for (u32 c = c0; c < c1; c++) s += B.cellCount[c];
-  s_sum[tid] = s;
-  __syncthreads();
-  for (u32 off = 1; off < (u32)kThreads; off <<= 1) {
-    const u32 add = tid >= off ? s_sum[tid - off] : 0u;
+  T.sync();
+  // exclusive scan by workgroup 0: a contiguous chunk of cells per thread, then
+  // the chunk sums
+  if (blockIdx.x == 0) {
+    const u32 tid = threadIdx.x, nt = blockDim.x;
+    const u32 chunk = (C + nt - 1) / nt;
+    const u32 c0 = min(tid * chunk, C), c1 = min(c0 + chunk, C);
+    u32 s = 0;
+    for (u32 c = c0; c < c1; c++) s += B.cellCount[c];
+    s_sum[tid] = s;
     __syncthreads();
-    s_sum[tid] += add;
-    __syncthreads();
+    for (u32 off = 1; off < nt; off <<= 1) {
+      const u32 add = tid >= off ? s_sum[tid - off] : 0u;
+      __syncthreads();
+      s_sum[tid] += add;
+      __syncthreads();
+    }
+    u32 run = s_sum[tid] - s;
+    for (u32 c = c0; c < c1; c++) {
+      B.cellStart[c] = run;
+      B.cursor[c] = run;
+      run += B.cellCount[c];
+    }
+    if (tid == 0) B.cellStart[C] = N;
   }
-  u32 run = s_sum[tid] - s;
-  for (u32 c = c0; c < c1; c++) {
-    B.cellStart[c] = run;
-    B.cursor[c] = run;
-    run += B.cellCount[c];
-  }
-  if (tid == 0) B.cellStart[C] = N;
-  __syncthreads();
-  for (u32 p = tid; p < N; p += kThreads) B.sorted[atomicAdd(&B.cursor[B.cellOf[p]], 1u)] = p;
-  __syncthreads();
-  for (u32 p = tid; p < N; p += kThreads) {
+  T.sync();
+  for (u32 p = T.rank; p < N; p += T.size) B.sorted[atomicAdd(&B.cursor[B.cellOf[p]], 1u)] = p;
+  T.sync();
+  for (u32 p = T.rank; p < N; p += T.size) {
     const d2 xp = B.x[p];
     const int cx = (int)(B.cellOf[p] % (u32)P.ncx), cy = (int)(B.cellOf[p] / (u32)P.ncx);
     const int x0 = max(cx - 1, 0), x1 = min(cx + 1, P.ncx - 1);
@@ -227,7 +248,7 @@ __device__ void buildNeighbours(const Prm& P, const Buf& B) {
     const m2 F = B.F[p];
     B.Vn[p] = P.V0 * (F.a * F.d - F.b * F.c);
   }
-  __syncthreads();
+  T.sync();
 }
 
 // sum_b (V_b (v_b - v_p)_i) gt_j with gt = L_p gradW(x_p - x_b) (xpbi2d.py:157-160)
@@ -250,12 +271,14 @@ __device__ __forceinline__ m2 trialF(const Prm& P, const m2& gv, const m2& F) {
   return mm(M, F);
 }
 
+template <bool kGrid>
 __global__ __launch_bounds__(kThreads) void k_run(Prm P, Buf B, int substeps) {
-  const u32 tid = threadIdx.x, N = P.N;
+  const Team<kGrid> T;
+  const u32 N = P.N;
   for (int step = 0; step < substeps; step++) {
-    buildNeighbours(P, B);
+    buildNeighbours(P, B, T);
     // Bonet-Lok L (xpbi2d.py:142-148), gravity, lambda = 0
-    for (u32 p = tid; p < N; p += kThreads) {
+    for (u32 p = T.rank; p < N; p += T.size) {
       const d2 xp = B.x[p];
       m2 A{0.0, 0.0, 0.0, 0.0};
       FOR_NBR(p, b) {
@@ -276,10 +299,10 @@ __global__ __launch_bounds__(kThreads) void k_run(Prm P, Buf B, int substeps) {
       B.v[p] = d2{v.x + P.dt * P.g0, v.y + P.dt * P.g1};
       B.lam[p] = 0.0;
     }
-    __syncthreads();
+    T.sync();
     for (int it = 0; it < P.iterations; it++) {
       // constraint of each particle (xpbi2d.py:155-182)
-      for (u32 p = tid; p < N; p += kThreads) {
+      for (u32 p = T.rank; p < N; p += T.size) {
         const d2 xp = B.x[p], vp = B.v[p];
         const m2 L = B.L[p], F = B.F[p];
         const Svd s = svd2(trialF(P, gradV(P, B, p, L, xp, vp), F));
@@ -313,9 +336,9 @@ __global__ __launch_bounds__(kThreads) void k_run(Prm P, Buf B, int substeps) {
         B.G[p] = G;
         B.gCp[p] = gcp;
       }
-      __syncthreads();
+      T.sync();
       // Jacobi velocity update (xpbi2d.py:183-187): gathered over the pairs (b, i)
-      for (u32 i = tid; i < N; i += kThreads) {
+      for (u32 i = T.rank; i < N; i += T.size) {
         const d2 xi = B.x[i];
         const double Vi = B.Vn[i];
         d2 acc{0.0, 0.0};
@@ -337,9 +360,9 @@ __global__ __launch_bounds__(kThreads) void k_run(Prm P, Buf B, int substeps) {
         B.v[i] = vn;
         B.xc[i] = d2{xi.x + P.dt * vn.x, xi.y + P.dt * vn.y};
       }
-      __syncthreads();
+      T.sync();
       // position correction (xpbi2d.py:189-201) and walls (xpbi2d.py:203-220)
-      for (u32 i = tid; i < N; i += kThreads) {
+      for (u32 i = T.rank; i < N; i += T.size) {
         const d2 xci = B.xc[i];
         d2 acc{0.0, 0.0};
         FOR_NBR(i, b) {  // pairs (i, b): +corr
@@ -395,10 +418,10 @@ __global__ __launch_bounds__(kThreads) void k_run(Prm P, Buf B, int substeps) {
         }
         B.v[i] = d2{vv[0], vv[1]};
       }
-      __syncthreads();
+      T.sync();
     }
     // XSPH (xpbi2d.py:222-225)
-    for (u32 p = tid; p < N; p += kThreads) {
+    for (u32 p = T.rank; p < N; p += T.size) {
       const d2 xp = B.x[p], vp = B.v[p];
       d2 acc{0.0, 0.0};
       FOR_NBR(p, b) {
@@ -411,29 +434,29 @@ __global__ __launch_bounds__(kThreads) void k_run(Prm P, Buf B, int substeps) {
       }
       B.dvx[p] = acc;
     }
-    __syncthreads();
-    for (u32 p = tid; p < N; p += kThreads) {
+    T.sync();
+    for (u32 p = T.rank; p < N; p += T.size) {
       const d2 v = B.v[p], a = B.dvx[p];
       B.v[p] = d2{v.x + P.xsphC * a.x, v.y + P.xsphC * a.y};
     }
-    __syncthreads();
+    T.sync();
     // F update with the return map (xpbi2d.py:227-233)
-    for (u32 p = tid; p < N; p += kThreads) {
+    for (u32 p = T.rank; p < N; p += T.size) {
       const m2 F = B.F[p];
       const Svd s = svd2(trialF(P, gradV(P, B, p, B.L[p], B.x[p], B.v[p]), F));
       double sp0, sp1;
       druckerPrager(P, s.s0, s.s1, &sp0, &sp1);
       B.F[p] = usv(s.U, sp0, sp1, s.V);
     }
-    __syncthreads();
+    T.sync();
     // positions (xpbi2d.py:235-237)
-    for (u32 p = tid; p < N; p += kThreads) {
+    for (u32 p = T.rank; p < N; p += T.size) {
       const d2 x = B.x[p], v = B.v[p];
       const double lo = 0.5 * P.h;
       B.x[p] = d2{fmin(fmax(x.x + P.dt * v.x, lo), P.dom0 - 0.5 * P.h),
                   fmin(fmax(x.y + P.dt * v.y, lo), P.dom1 - 0.5 * P.h)};
     }
-    __syncthreads();
+    T.sync();
   }
 }
 
@@ -447,6 +470,8 @@ struct xpbi2d_sim {
   Prm P{};
   Buf B{};
   int dev = 0;
+  int single = 0;     // 1: one workgroup; 0: a cooperative grid, one thread per particle
+  int coopBlocks = 0;  // blocks of the cooperative grid (capped by co-residency)
   hipStream_t stream = nullptr;
   std::string err;
   void* mem = nullptr;
@@ -562,7 +587,24 @@ int xpbi2d_create(uint32_t N, const double* positions, const xpbi2d_params* p, i
     delete s;
     return XPBI_ERR_HIP;
   }
+  // the cooperative grid: one 256-thread workgroup per 256 particles, no more
+  // than can be resident at once (the grid barrier needs every block running)
+  int coop = 0, cus = 0, perCU = 0;
+  hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device);
+  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+  hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, (const void*)k_run<true>, 256, 0);
+  const int want = (int)((N + 255) / 256);
+  s->coopBlocks = want < perCU * cus ? want : perCU * cus;
+  s->single = (coop && s->coopBlocks > 0) ? 0 : 1;
   *out = s;
+  return XPBI_OK;
+}
+
+int xpbi2d_set_workgroups(xpbi2d_sim* s, int single) {
+  if (!s) return XPBI_ERR_ARG;
+  if (single != 0 && single != 1) return fail2(s, XPBI_ERR_ARG, "single must be 0 or 1");
+  if (!single && s->coopBlocks <= 0) return fail2(s, XPBI_ERR_STATE, "cooperative launch unavailable");
+  s->single = single;
   return XPBI_OK;
 }
 
@@ -587,8 +629,17 @@ static int run2d(xpbi2d_sim* s, int substeps, float* ms) {
     hipEventCreate(&b);
     hipEventRecord(a, s->stream);
   }
-  if (substeps > 0) hipLaunchKernelGGL(k_run, dim3(1), dim3(kThreads), 0, s->stream, s->P, s->B, substeps);
-  hipError_t e = hipGetLastError();
+  hipError_t e = hipSuccess;
+  if (substeps > 0) {
+    if (s->single) {
+      hipLaunchKernelGGL(k_run<false>, dim3(1), dim3(kThreads), 0, s->stream, s->P, s->B, substeps);
+    } else {
+      void* args[] = {(void*)&s->P, (void*)&s->B, (void*)&substeps};
+      e = hipLaunchCooperativeKernel((const void*)k_run<true>, dim3(s->coopBlocks), dim3(256), args, 0, s->stream);
+      if (e != hipSuccess) return hipfail2(s, e, "cooperative launch of k_run");
+    }
+  }
+  e = hipGetLastError();
   if (e != hipSuccess) return hipfail2(s, e, "k_run launch");
   if (ms) hipEventRecord(b, s->stream);
   if ((e = hipStreamSynchronize(s->stream)) != hipSuccess) return hipfail2(s, e, "hipStreamSynchronize");

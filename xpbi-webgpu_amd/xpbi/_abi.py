@@ -127,6 +127,7 @@ SIGNATURES = {
     "xpbi2d_last_error": (C.c_char_p, [_P]),
     "xpbi2d_step": (C.c_int, [_P, C.c_int]),
     "xpbi2d_time": (C.c_int, [_P, C.c_int, C.POINTER(C.c_float)]),
+    "xpbi2d_set_workgroups": (C.c_int, [_P, C.c_int]),
     "xpbi2d_read": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double), C.c_size_t]),
     "xpbi2d_write": (C.c_int, [_P, C.c_int, C.POINTER(C.c_double), C.c_size_t]),
 }

@@ -90,6 +90,11 @@ class Prototype2D:
     def step(self, substeps=1):
         _check(lib().xpbi2d_step(self._h, int(substeps)), self._h, "xpbi2d_step")
 
+    def set_workgroups(self, single):
+        """single=True: one workgroup runs the scene; False: a cooperative grid, one
+        thread per particle (the default).  Results are identical."""
+        _check(lib().xpbi2d_set_workgroups(self._h, int(bool(single))), self._h, "xpbi2d_set_workgroups")
+
     def time(self, substeps):
         """Milliseconds of `substeps` substeps (one launch, HIP events)."""
         ms = C.c_float()

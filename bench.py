@@ -51,7 +51,7 @@ def to_residual(sim, snap, params, tol, cap, substeps, before=None):
     indexed by particle id."""
     import numpy as np
     sim.set_warmstart(params)
-    sim.set_tolerance(tol, 4)
+    sim.set_tolerance(tol, 1)
     its = []
     for k, v in snap.items():
         sim.write(k, v)

@@ -263,8 +263,11 @@ int xpbi_read_residuals(xpbi_sim* sim, double* out, int cap, int* count);
  * particles (the numerator of Delta-lambda / omega, simkernels.js:305) and is the
  * last one applied once r_k <= tol; the test runs on the device, so a substep
  * needs no host synchronisation.  check_every > 0 also lets the host stop
- * enqueuing iterations: it polls the device flag every check_every iterations
- * (one stream synchronisation each).  tol = 0 restores the fixed count. */
+ * enqueuing iterations: every check_every iterations it reads the device's
+ * report (written into pinned host memory, no stream synchronisation) and waits
+ * there until the device is at most 2 iterations behind, so that launches past
+ * the last iteration (which return at entry) stay few.  tol = 0 restores the
+ * fixed count. */
 int xpbi_set_tolerance(xpbi_sim* sim, float tol, int check_every);
 /* Jacobi iterations applied by the last substep (a learned first iteration
  * counts as one) and, with tol > 0, its final residual r_k (-1 otherwise).

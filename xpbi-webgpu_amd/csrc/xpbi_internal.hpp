@@ -35,6 +35,9 @@ struct PSetC {
 // constraints of iteration 0.  Once r_k <= tol, iteration k is the last one
 // applied: later launches of the loop exit at once.
 struct SolveCtl {
+  // done and iters form one 8-byte word, written and read whole: the test of
+  // iteration k runs in the first block of that iteration's solveB while the
+  // kernel's other blocks read the pair
   u32 done;
   u32 iters;  // Jacobi iterations applied
   u32 c0preset;  // c0sq was set by the warm-start features pass
@@ -56,7 +59,9 @@ struct CtlMirror {
 // iteration k of the loop is skipped once an earlier iteration was the last:
 // the stopping iteration itself (iters == k + 1) still runs its solveB / PBF
 __device__ __forceinline__ bool skipIteration(const SolveCtl* ctl, u32 k) {
-  return ctl && ctl->done && ctl->iters <= k;
+  if (!ctl) return false;
+  const uint2 di = *reinterpret_cast<const uint2*>(ctl);  // (done, iters)
+  return di.x && di.y <= k;
 }
 
 constexpr int kMaxProfileMarks = 4096;
@@ -129,7 +134,7 @@ struct Sim {
   int checkEvery = 4;       // host polls the done flag every checkEvery iterations (0: never)
   SolveCtl* ctl = nullptr;  // device
   SolveCtl* ctlHost = nullptr;  // pinned copy of ctl
-  CtlMirror* mirror = nullptr;  // pinned, written by k_resid_final
+  CtlMirror* mirror = nullptr;  // pinned, written by the residual test in solveB
   u32 solveGen = 0;             // generation of the current residual loop
   int lastIters = 0;        // fixed-iteration count of the last substep (tol == 0)
   int warmIters = 0;        // 1 when the last substep's first iteration was the learned one

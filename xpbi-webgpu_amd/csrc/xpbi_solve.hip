@@ -139,40 +139,51 @@ __global__ __launch_bounds__(256) void k_solveA(const KP* __restrict__ kps, Work
   if (part) blockReduce2((double)r2, (double)c2, part + 2 * blockIdx.x);
 }
 
-// deterministic second stage of the residual reduction; with ctl, also the
-// stopping test of iteration k (skipped once the loop is done)
-__global__ __launch_bounds__(256) void k_resid_final(const double* __restrict__ part, u32 nb, double* __restrict__ out,
-                                                     SolveCtl* __restrict__ ctl, u32 k, double tol2,
-                                                     CtlMirror* __restrict__ mirror, u32 gen) {
-  if (skipIteration(ctl, k)) return;
+// the residual test of iteration k, run by the first block of that iteration's
+// solveB (the partials of its solveA are complete once solveB starts; only
+// solveA of iteration k + 1 needs the outcome): the deterministic second stage
+// of the reduction, the residual history entry (out) and, with ctl, the
+// stopping test and the host's mirror
+struct ResidTest {
+  const double* part;  // 2 doubles per block of solveA; null: no test
+  u32 nb;
+  double* out;
+  SolveCtl* ctl;
+  double tol2;
+  CtlMirror* mirror;
+  u32 gen;
+};
+__device__ __forceinline__ void residTest(const ResidTest& t, u32 k) {
   __shared__ double s_out[2];
   double a = 0.0, b = 0.0;
-  for (u32 j = threadIdx.x; j < nb; j += 256u) {
-    a += part[2 * j];
-    b += part[2 * j + 1];
+  for (u32 j = threadIdx.x; j < t.nb; j += 256u) {
+    a += t.part[2 * j];
+    b += t.part[2 * j + 1];
   }
   blockReduce2(a, b, s_out);
   __syncthreads();
   if (threadIdx.x == 0) {
-    if (out) {
-      out[0] = s_out[0];
-      out[1] = s_out[1];
+    if (t.out) {
+      t.out[0] = s_out[0];
+      t.out[1] = s_out[1];
     }
+    SolveCtl* ctl = t.ctl;
     if (ctl) {
       if (k == 0 && !ctl->c0preset) ctl->c0sq = s_out[1];
       const double c0 = ctl->c0sq;
-      ctl->iters = k + 1;
+      const u32 done = (c0 <= 0.0 || s_out[0] <= t.tol2 * c0) ? 1u : 0u;
       ctl->r2 = s_out[0];
-      if (c0 <= 0.0 || s_out[0] <= tol2 * c0) ctl->done = 1u;
-      if (mirror) {
-        volatile CtlMirror* m = mirror;
+      *reinterpret_cast<uint2*>(ctl) = make_uint2(done, k + 1u);
+      if (t.mirror) {
+        volatile CtlMirror* m = t.mirror;
         m->iters = k + 1;
-        m->done = ctl->done;
+        m->done = done;
         __threadfence_system();
-        m->gen = gen;
+        m->gen = t.gen;
       }
     }
   }
+  __syncthreads();  // s_out is read above before a later call may write it
 }
 
 __global__ void k_ctl_reset(SolveCtl* __restrict__ ctl) {
@@ -200,9 +211,10 @@ template <bool kPersist>
 __global__ __launch_bounds__(256) void k_solveB(const KP* __restrict__ kps, WorkTab W, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
                                                 const float4* __restrict__ vin, M3C Sp,
                                                 const float4* __restrict__ selfDv, const u32* __restrict__ cs,
-                                                NLc nl, float4* __restrict__ vout, const SolveCtl* __restrict__ ctl,
-                                                u32 k, float4* __restrict__ xcOut) {
+                                                NLc nl, float4* __restrict__ vout, const SolveCtl* ctl,
+                                                u32 k, float4* __restrict__ xcOut, ResidTest rt) {
   if (skipIteration(ctl, k)) return;
+  if (rt.part && blockIdx.x == 0) residTest(rt, k);
   __shared__ MatLds T;
   u32 curScene = ~0u;
   forWork<kPersist>(W, [&](u32 scene, u32 first) {
@@ -530,6 +542,8 @@ __global__ __launch_bounds__(256) void k_collect_targets(const float* __restrict
 // ---------------------------------------------------------------- schedule
 static inline u32 cdiv(u64 a, u64 b) { return (u32)((a + b - 1) / b); }
 static inline M3C cm(const M3P& m) { return M3C{m.a, m.b, m.c}; }
+// iterations the host enqueues past the last residual test it has seen
+constexpr int kPollLead = 2;
 
 // sim.js:255-266 after the reorder; ends with s->vcur on the final velocity
 // the device's report for the current loop (generation gen): true once it has
@@ -595,7 +609,8 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
       s->prof_mark("mlp");
       XPBI_LAUNCH_W(k_solveB, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
                          (const float4*)s->vel[vi], cm(s->Sp), (const float4*)s->selfDv, (const u32*)s->cellStart,
-                         nl, s->vel[vi ^ 1], (const SolveCtl*)nullptr, 0u, water ? s->xc : (float4*)nullptr);
+                         nl, s->vel[vi ^ 1], (const SolveCtl*)nullptr, 0u, water ? s->xc : (float4*)nullptr,
+                         ResidTest{});
       s->prof_mark("solveB");
       vi ^= 1;
       if (water) {
@@ -614,10 +629,11 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
   int launched = 0;
   for (int k = 0; k < iterations - warm; k++) {
     if (loop && s->checkEvery > 0 && k > 0 && k % s->checkEvery == 0) {
-      // stop enqueuing once the device has flagged convergence; stay at most two
-      // check intervals ahead of the iterations whose test the device has run
-      // (the queued ones keep it busy meanwhile; skipped ones return at entry)
-      if (polledDone(s, gen, k - 2 * s->checkEvery)) break;
+      // stop enqueuing once the device has flagged convergence; stay at most
+      // kPollLead iterations ahead of the iterations whose test the device has
+      // run (the queued ones keep it busy meanwhile, about 0.25 ms each at 1M
+      // particles; launches past the last iteration return at entry)
+      if (polledDone(s, gen, k - kPollLead)) break;
     }
     double* out = (s->residOn && k < kMaxResidIters) ? s->residOut + 2 * k : nullptr;
     double* part = (out || loop) ? s->residPart : nullptr;
@@ -625,14 +641,11 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
                        (const float4*)s->vel[vi], cm(F), cm(s->L), S.aux, s->Sp, s->selfDv,
                        (const u32*)s->cellStart, nl, part, (const SolveCtl*)ctl, (u32)k, hbHint);
     s->prof_mark("solveA");
-    if (part) {
-      hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(256), 0, st, (const double*)part, gN, out, ctl, (u32)k, tol2,
-                         loop ? s->mirror : (CtlMirror*)nullptr, gen);
-      if (out) s->residIters = k + 1;
-    }
+    if (out) s->residIters = k + 1;
+    const ResidTest rt{(const double*)part, gN, out, ctl, tol2, loop ? s->mirror : (CtlMirror*)nullptr, gen};
     XPBI_LAUNCH_W(k_solveB, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
                        (const float4*)s->vel[vi], cm(s->Sp), (const float4*)s->selfDv, (const u32*)s->cellStart,
-                       nl, s->vel[vi ^ 1], (const SolveCtl*)ctl, (u32)k, water ? s->xc : (float4*)nullptr);
+                       nl, s->vel[vi ^ 1], (const SolveCtl*)ctl, (u32)k, water ? s->xc : (float4*)nullptr, rt);
     s->prof_mark("solveB");
     vi ^= 1;
     if (water) {

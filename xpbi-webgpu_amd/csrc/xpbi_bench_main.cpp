@@ -117,9 +117,17 @@ int main(int argc, char** argv) {
   check(xpbi_time_substeps(sim, steps, iters, 0, &ms), sim, "xpbi_time_substeps");
   float cms = 0.0f;
   if (copyMiB > 0) check(xpbi_copy_bandwidth(sim, (size_t)copyMiB << 20, 1, &cms), sim, "xpbi_copy_bandwidth");
+  // FNV-1a over the final positions and velocities: equal hashes from two builds
+  // mean bit-identical states
+  uint64_t hash = 1469598103934665603ull;
+  std::vector<uint8_t> buf(16ull * N);
+  for (int field : {XPBI_FIELD_POS, XPBI_FIELD_VEL}) {
+    check(xpbi_read(sim, field, buf.data(), buf.size()), sim, "xpbi_read");
+    for (uint8_t byte : buf) hash = (hash ^ byte) * 1099511628211ull;
+  }
   std::printf("{\"particles\": %u, \"settle\": %d, \"steps\": %d, \"iterations\": %d, \"ms_per_substep\": %.4f, "
-              "\"copy_mib\": %d, \"copy_ms\": %.4f}\n",
-              N, settle, steps, iters, ms / (steps > 0 ? steps : 1), copyMiB, cms);
+              "\"copy_mib\": %d, \"copy_ms\": %.4f, \"state_fnv1a\": \"%016llx\"}\n",
+              N, settle, steps, iters, ms / (steps > 0 ? steps : 1), copyMiB, cms, (unsigned long long)hash);
   xpbi_destroy(sim);
   return 0;
 }

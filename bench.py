@@ -217,7 +217,8 @@ def poke_scene(params, train, settle, substeps, cap):
     out = learned_vs_plain(sim, 1e-4, cap, substeps, params, train, before)
     out.update(particles=sim.N, settle_substeps=settle, collider=dict(R=col["R"], speed_m_s=1.0),
                note="collider moved every substep; learned iterations count the learned first iteration; "
-                    "network trained on the DP sand columns only")
+                    "network trained on the two DP sand columns and a 125k von Mises poke at r = 0.01 (another "
+                    "resolution, size and seed than this block)")
     out.pop("training", None)
     sim.close()
     return out
@@ -373,7 +374,8 @@ def main():
                    "particles": N, "dt": scene["dt"], "settle_substeps": args.settle,
                    "residual": 1e-4 if resid else None, "iteration_cap": args.residual_cap if resid else I,
                    "timing": ("eager substeps; iterations stop on the device-side residual test, which the "
-                              "host polls every 4 iterations without stalling the stream; no host read between "
+                              "host reads every iteration from pinned memory without stalling the stream, staying at most 2 "
+                              "iterations ahead; no host read between "
                               "timed substeps (the iteration counts come from an identical untimed pass); plain "
                               "and learned from the same snapshot" if resid else
                               "HIP-graph replay of two captured substeps, autotuned launch shape"),

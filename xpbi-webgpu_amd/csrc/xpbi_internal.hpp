@@ -120,7 +120,10 @@ struct Sim {
   // the cell rows)
   uint16_t* nbrLst = nullptr;
   uint2* nbrHead = nullptr;
-  u32 nbrK = 128;  // 256 B per particle: compressed water reaches ~110 neighbours (q99)
+  // 512 B per particle.  Clumping PBF water in the 1M dam breach has 110 neighbours
+  // at q99 and up to ~200; a particle past nbrK walks all its cell rows (800+
+  // candidates in the densest cells), and its whole wave waits for it
+  u32 nbrK = 256;
   float4* xc = nullptr;  // water scenes: predicted positions x + dt v (w = tag) after each solveB
   bool sorted = false;  // a substep has run (cellStart/perm valid)
 

@@ -158,6 +158,7 @@ struct MatLds {
   float mass[8];
   float invMass[8];
   float wc[8][8];  // contact weight m_b / (m_a + m_b) of material pair (a, b) (simkernels.js:346)
+  float mRho[8][8];  // mass of material b over density of material a (simkernels.js:429, :470)
   u32 solidMask;   // bit k set when material k is not water
 };
 // every material of the scene is solid: the per-neighbour phase tests vanish
@@ -173,6 +174,7 @@ __device__ __forceinline__ void loadMatLds(MatLds& t, const DevMat* __restrict__
   if (tid < 64) {
     const float ma = mats[tid >> 3].mass, mb = mats[tid & 7u].mass;
     t.wc[tid >> 3][tid & 7u] = mb / (ma + mb);
+    t.mRho[tid >> 3][tid & 7u] = mb / mats[tid >> 3].density;
   }
   if (tid == 0) {
     u32 mask = 0;

@@ -257,6 +257,7 @@ __global__ __launch_bounds__(256) void k_pbfA(const KP* __restrict__ kps, WorkTa
     const u32 mi = matOfTag(pi.w);
     if ((T.solidMask >> mi) & 1u) { aux[i].w = 0.0f; return; }
     const float rho0 = T.m[mi].density;
+    const float* mRho = T.mRho[mi];  // m_b / rho0 by the neighbour's material
     const f3 xn = xyz(pi);
     const f3 xci = xyz(xc[i]);
     float rho = T.mass[mi] * P.poly6_0;
@@ -267,9 +268,10 @@ __global__ __launch_bounds__(256) void k_pbfA(const KP* __restrict__ kps, WorkTa
       if (b != i) {
         f3 dxv = xci - xyz(xb);
         if (dot(dxv, dxv) < P.supp2) {
-          float mb = T.mass[matOfTag(xb.w)];
+          const u32 kb = matOfTag(xb.w);
+          float mb = T.mass[kb];
           rho = rho + mb * kernelPoly6(P, len(dxv));
-          f3 g = kernelSpikyGrad(P, dxv) * (mb / rho0);
+          f3 g = kernelSpikyGrad(P, dxv) * mRho[kb];
           gradSum = gradSum + g;
           grad2 = grad2 + dot(g, g);
         }
@@ -315,7 +317,7 @@ __global__ __launch_bounds__(256) void k_pbfB(const KP* __restrict__ kps, WorkTa
             float ratio = w / wmax(P.w0, 1e-12f);
             float r2 = ratio * ratio;
             float sCorr = -P.sCorrK * (r2 * r2);
-            dp = dp + (li + lb + sCorr) * kernelSpikyGrad(P, dxv) * (T.m[wk].mass / T.m[wk].density);
+            dp = dp + (li + lb + sCorr) * kernelSpikyGrad(P, dxv) * T.mRho[wk][wk];
           }
         }
       }

@@ -173,6 +173,14 @@ int xpbi_write_scene_params(xpbi_sim* sim, int scene, const xpbi_params* p);
  * grid of that many blocks (rounded up to a multiple of 8, at most one per
  * entry) in which each XCD walks a contiguous eighth of the entries. */
 int xpbi_set_launch(xpbi_sim* sim, int blocks);
+
+/* Slots of the support-neighbour list (no reference counterpart: the reference
+ * walks the 9 cell rows in every sweep, web/js/shaders/simkernels.js:59-74, and
+ * this build records the neighbours once per substep instead).  Default 256, at
+ * 2 bytes per slot per particle; 0..1024.  A particle with more neighbours walks
+ * its cell rows, and slots = 0 walks rows everywhere and frees the list: the
+ * results are the same bit for bit, only the speed and the memory differ. */
+int xpbi_set_neighbour_slots(xpbi_sim* sim, int slots);
 int xpbi_get_launch(const xpbi_sim* sim, int* blocks, int* work_entries);
 
 /* substeps x substep like xpbi_step, replayed from a HIP graph of two captured

@@ -63,9 +63,11 @@ def test_argument_errors_without_device():
     d.domainMax[0] = -1.0
     assert lib.xpbi_create(ctypes.byref(d), ctypes.byref(h)) == xpbi._abi.ERR_ARG
     assert h.value is None
-    for f in ("xpbi_substep", "xpbi_sync", "xpbi_grid_info", "xpbi_write_params", "xpbi_read"):
+    for f in ("xpbi_substep", "xpbi_sync", "xpbi_grid_info", "xpbi_write_params", "xpbi_read",
+              "xpbi_set_neighbour_slots"):
         args = {"xpbi_substep": (None, 1, 0), "xpbi_sync": (None,), "xpbi_write_params": (None, None),
-                "xpbi_grid_info": (None, None, None, None, None, None), "xpbi_read": (None, 0, None, 0)}[f]
+                "xpbi_grid_info": (None, None, None, None, None, None), "xpbi_read": (None, 0, None, 0),
+                "xpbi_set_neighbour_slots": (None, 16)}[f]
         assert getattr(lib, f)(*args) == xpbi._abi.ERR_ARG
 
 
@@ -125,5 +127,5 @@ def test_napi_addon_loads():
     for n in ("create", "destroy", "writeMaterials", "writeParams", "step", "read", "write", "profileSubstep",
               "timeSubsteps", "gridInfo", "sync", "createBatch", "sceneInfo", "writeSceneMaterials",
               "writeSceneParams", "setTolerance", "lastIterations", "setWarmstart", "graphStep", "setLaunch",
-              "getLaunch", "autotune"):
+              "getLaunch", "autotune", "setNeighbourSlots"):
         assert n in names

@@ -811,6 +811,30 @@ int xpbi_get_launch(const xpbi_sim* h, int* blocks, int* work_entries) {
   return XPBI_OK;
 }
 
+int xpbi_set_neighbour_slots(xpbi_sim* h, int slots) {
+  if (!h) return XPBI_ERR_ARG;
+  Sim* s = &h->s;
+  if (slots < 0 || slots > (int)Sim::kMaxNbrSlots) return fail(s, XPBI_ERR_ARG, "slots must be in 0..1024");
+  if ((u32)slots == s->nbrK) return XPBI_OK;
+  int r = xpbi_sync(h);
+  if (r) return r;
+  hipSetDevice(s->dev);
+  drop_graph(s);  // a captured graph holds the old list
+  for (auto it = s->allocs.begin(); it != s->allocs.end(); ++it) {
+    if (*it == (void*)s->nbrLst) {
+      s->allocs.erase(it);
+      break;
+    }
+  }
+  hipFree(s->nbrLst);
+  s->nbrLst = nullptr;
+  s->nbrK = 0;
+  hipError_t e = dalloc(s, &s->nbrLst, (size_t)slots * s->N);
+  if (e != hipSuccess) return hip_fail(s, e, "hipMalloc (neighbour list)");
+  s->nbrK = (u32)slots;
+  return XPBI_OK;
+}
+
 int xpbi_graph_step(xpbi_sim* h, int substeps, int iterations, int has_water) {
   if (!h) return XPBI_ERR_ARG;
   Sim* s = &h->s;

@@ -124,6 +124,7 @@ struct Sim {
   // at q99 and up to ~200; a particle past nbrK walks all its cell rows (800+
   // candidates in the densest cells), and its whole wave waits for it
   u32 nbrK = 256;
+  static constexpr u32 kMaxNbrSlots = 1024;
   float4* xc = nullptr;  // water scenes: predicted positions x + dt v (w = tag) after each solveB
   bool sorted = false;  // a substep has run (cellStart/perm valid)
 

@@ -94,7 +94,7 @@ __global__ __launch_bounds__(256) void k_computeL(const KP* __restrict__ kps, Wo
     }
     // the particle's own offset must fit too (the sweeps' batch tails gather i)
     flag |= (i - base) > kNbrOffMax ? kNbrFlag : 0u;
-    const u32 ns = min(n, kStage);
+    const u32 ns = min(n, min(kStage, K));  // the list may hold fewer slots than the stage
     for (u32 k = 0; k < ns; k++) nLst[(size_t)k * NT + i] = stage[k][tid];
     nHead[i] = make_uint2(base, n | flag);
     SVD3<double> s = svd3<double>(mcast<double>(A));

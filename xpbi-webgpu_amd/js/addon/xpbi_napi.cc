@@ -512,6 +512,17 @@ napi_value SetLaunch(napi_env env, napi_callback_info info) {
   if (rc) return throw_xpbi(env, s, rc, "xpbi_set_launch");
   return nullptr;
 }
+// setNeighbourSlots(handle, slots)
+napi_value SetNeighbourSlots(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value a[2];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
+  xpbi_sim* s = get_sim(env, a[0]);
+  if (!s) return nullptr;
+  int rc = xpbi_set_neighbour_slots(s, arg_i32(env, a[1]));
+  if (rc) return throw_xpbi(env, s, rc, "xpbi_set_neighbour_slots");
+  return nullptr;
+}
 napi_value GetLaunch(napi_env env, napi_callback_info info) {
   size_t argc = 1;
   napi_value a[1];
@@ -561,7 +572,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"createBatch", CreateBatch}, {"sceneInfo", SceneInfo}, {"writeSceneMaterials", WriteSceneMaterials},
       {"writeSceneParams", WriteSceneParams}, {"setTolerance", SetTolerance}, {"lastIterations", LastIterations},
       {"setWarmstart", SetWarmstart}, {"graphStep", GraphStep}, {"setLaunch", SetLaunch},
-      {"getLaunch", GetLaunch}, {"autotune", Autotune},
+      {"getLaunch", GetLaunch}, {"autotune", Autotune}, {"setNeighbourSlots", SetNeighbourSlots},
   };
   for (auto& f : fns) {
     napi_value fn;

@@ -193,6 +193,7 @@ class Simulation {
   // substeps replayed from a captured HIP graph (same results as step)
   graphStep(substeps, iterations, hasWater) { loadAddon().graphStep(this.handle, substeps, iterations, !!hasWater); }
   setLaunch(blocks) { loadAddon().setLaunch(this.handle, blocks); }
+  setNeighbourSlots(slots) { loadAddon().setNeighbourSlots(this.handle, slots); }
   getLaunch() { return loadAddon().getLaunch(this.handle); }
   // times the launch shapes on the live state, keeps the fastest: {blocks, ms}
   autotune(iterations, hasWater, substeps) { return loadAddon().autotune(this.handle, iterations, !!hasWater, substeps || 2); }

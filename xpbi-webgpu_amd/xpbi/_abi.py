@@ -104,6 +104,7 @@ SIGNATURES = {
     "xpbi_write_scene_materials": (C.c_int, [_P, C.c_int, C.POINTER(Material), C.c_int]),
     "xpbi_write_scene_params": (C.c_int, [_P, C.c_int, C.POINTER(Params)]),
     "xpbi_set_launch": (C.c_int, [_P, C.c_int]),
+    "xpbi_set_neighbour_slots": (C.c_int, [_P, C.c_int]),
     "xpbi_get_launch": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "xpbi_graph_step": (C.c_int, [_P, C.c_int, C.c_int, C.c_int]),
     "xpbi_autotune": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_float)]),

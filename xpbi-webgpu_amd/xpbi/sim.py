@@ -264,6 +264,11 @@ class Simulation:
         """0: one block per 256-particle run; > 0: persistent grid of that many blocks."""
         check(lib().xpbi_set_launch(self._h, int(blocks)), self._h, "set_launch")
 
+    def set_neighbour_slots(self, slots):
+        """Slots of the support-neighbour list (default 256; 0 walks the cell rows in
+        every sweep, like the reference): same results, other speed and memory."""
+        check(lib().xpbi_set_neighbour_slots(self._h, int(slots)), self._h, "set_neighbour_slots")
+
     def get_launch(self):
         b, n = C.c_int(), C.c_int()
         check(lib().xpbi_get_launch(self._h, C.byref(b), C.byref(n)), self._h)

@@ -27,13 +27,18 @@
 //   rmSnowClamp / snow state  clamp bounds and the Jp update identity
 //   rmNACC                    yield value y <= 1e-4 after a case-3 projection,
 //                             tip formulas for cases 1/2
-//   solveA/solveB (Delta-lam) closed-form two-particle case, momentum neutrality
-//                             of the constraint gather (sum m dv = 0 without
-//                             walls/contacts), rest lattice gives C = 0
-//   substep                   free fall without walls: v = g t, F = I; a rigid
-//                             translation of a lattice gives no constraint impulse
-//   pbfA/pbfB, xsph           pbf: rest-spacing water gives lambda = 0 below rho0;
-//                             xsph of a uniform velocity field is zero
+//   solveA/solveB, xsph       (test_oracle_substep.py) a rest lattice has C = 0,
+//                             zero Sp and stays exactly at rest; a substep away
+//                             from walls keeps the total momentum (constraint
+//                             gather, XSPH and contacts are pairwise); dropping
+//                             selfDv, flipping the gather's gradient or XSPH's
+//                             velocity difference each fails one of these
+//   substep                   free fall: every particle has the same velocity,
+//                             the sum of the dt g steps, and F = I (f32 and f64);
+//                             a rigid translation gets no impulse at all
+//   pbfA/pbfB                 water below rho0 (checked with the oracle's own
+//                             poly6) gets lambda_pbf = 0 and, without s_corr,
+//                             stays at rest
 // Parity of this oracle with the reference WGSL *as executed by a WebGPU stack*
 // is unpinned: no WebGPU implementation exists in this environment.
 #include "xpbi_oracle.hpp"

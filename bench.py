@@ -20,7 +20,7 @@ Also reported: the fixed 20-iteration throughput replayed from a HIP graph under
 the autotuned launch shape; per-kernel HIP-event times of an eager run; the HBM
 roofline of the dominant kernel (solveA, with its HBM traffic from the committed
 counter profile); the 64-scene batched ensemble; the mixed water + sand scene;
-the von Mises poke to residual; the 2D fp64 column of the prototype (config 1)
+the von Mises poke to residual; the prototype's 2D fp64 40x80 column
 beside its numpy oracle; and the f32 C++ oracle's substep on the host cores (a
 reported baseline).
 """
@@ -132,11 +132,11 @@ def cpu_baseline(sim, scene, iterations, threads):
                        f"f32 C++ oracle, OpenMP {threads} threads, {dt:.1f} s")
 
 
-def proto2d_c1(with_cpu):
-    """Config 1: the 2D fp64 column of the prototype (40x80 at r = 0.01 m), its 300
+def column_2d(with_cpu):
+    """The prototype's 2D fp64 column (40x80 at r = 0.01 m): its 300
     substeps in one launch of the 2D engine; beside it the numpy oracle's substep."""
     from xpbi import proto2d
-    p, x = proto2d.c1_params(), proto2d.c1_positions()
+    p, x = proto2d.column40x80_params(), proto2d.column40x80_positions()
     warm = proto2d.Prototype2D(x, p)
     warm.step(5)
     warm.close()
@@ -150,7 +150,7 @@ def proto2d_c1(with_cpu):
     if with_cpu:
         sys.path.insert(0, os.path.join(REPO, "oracle"))
         import proto2d as o2
-        prm, st = o2.c1_params(), o2.c1_state()
+        prm, st = o2.column40x80_params(), o2.column40x80_state()
         t0 = time.perf_counter()
         o2.run(prm, st, 20)
         dt = time.perf_counter() - t0
@@ -160,10 +160,10 @@ def proto2d_c1(with_cpu):
 
 
 def mixed_scene(settle, steps):
-    """Config 3: dam breach, 600k PBF water beside a 400k DP sand wall (phi 35), 10 iterations."""
+    """Dam breach: 600k PBF water beside a 400k DP sand wall (phi 35), 10 iterations."""
     import xpbi
     from xpbi import scenes
-    sc = scenes.config_c3()
+    sc = scenes.dam_breach_1m()
     I = sc["iterations"]
     sim = xpbi.Simulation(sc, device=0)
     sim.writeParams(xpbi.params_from_scene(sc))
@@ -179,7 +179,7 @@ def mixed_scene(settle, steps):
     kern = {k: round(v[0] / max(v[1], 1) * 1e3, 1) for k, v in per.items()}
     g = ("computeL", "solveA", "solveB", "pbfA", "pbfB", "xsph", "updateF")
     g_ms = sum(per[k][0] for k in g if k in per) / 4
-    g_bytes = (264 + 348 * I) * N  # SURVEY upper bound of the gather kernels with water
+    g_bytes = (264 + 348 * I) * N  # compulsory bytes of the gather kernels with water (upper bound)
     out = dict(particles=N, water=int((sc["matIds"] == 1).sum()), sand=int((sc["matIds"] == 0).sum()),
                iterations=I, settle_substeps=settle, ms_per_substep_graph=round(ms, 4),
                particle_substeps_per_s=round(N / (ms / 1e3), 1), kernels_us=kern,
@@ -190,11 +190,11 @@ def mixed_scene(settle, steps):
 
 
 def poke_scene(params, train, settle, substeps, cap):
-    """Config 5: von Mises block 100^3 (sigmaY 5e3, E 2e5) poked by a 0.05 m sphere moving
+    """A von Mises block 100^3 (sigmaY 5e3, E 2e5) poked by a 0.05 m sphere moving
     down at 1 m/s (its position updated every substep): learned vs plain Jacobi to 1e-4."""
     import xpbi
     from xpbi import scenes
-    sc = scenes.config_c5()
+    sc = scenes.vm_poke_1m()
     sim = xpbi.Simulation(sc, device=0)
     col = sc["collider"]
     dt = sc["dt"]
@@ -229,7 +229,7 @@ def ensemble(I, settle, steps):
     import numpy as np
     import xpbi
     from xpbi import scenes
-    sl = scenes.config_c4()
+    sl = scenes.ensemble_64()
     sim = xpbi.Simulation(sl, device=0)
     for q, sc in enumerate(sl):
         sim.writeParams(xpbi.params_from_scene(sc), scene=q)
@@ -266,22 +266,22 @@ def main():
     ap.add_argument("--weights", default=None, help="warm-start network (.npy) instead of training in the run")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-residual", action="store_true")
-    ap.add_argument("--config", default="headline", choices=["headline", "c2"])
+    ap.add_argument("--config", default="column1m", choices=["column1m", "column128k"])
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-configs", action="store_true", help="skip the mixed (c3), poke (c5) and 2D (c1) scenes")
-    ap.add_argument("--only-config1", action="store_true", help="run only the 2D column (config 1); print its object")
+    ap.add_argument("--only-2d", action="store_true", help="run only the 2D 40x80 fp64 column; print its object")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: the substep engine has no multi-GPU path in this build "
-              "(each GPU box has one MI355X); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: a simulation runs on one device and the engine has no "
+              "multi-GPU path; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    if args.only_config1:
-        print(json.dumps(proto2d_c1(not args.no_cpu)))
+    if args.only_2d:
+        print(json.dumps(column_2d(not args.no_cpu)))
         return
 
     import xpbi
     from xpbi import scenes
-    scene = scenes.config_headline() if args.config == "headline" else scenes.config_c2()
+    scene = scenes.sand_column_1m() if args.config == "column1m" else scenes.sand_column_128k()
     I = args.iterations
     sim = xpbi.Simulation(scene, device=0)
     sim.writeParams(xpbi.params_from_scene(scene))
@@ -354,8 +354,8 @@ def main():
         ens = ensemble(I, 60, 20)
     mixed = poke = c1 = None
     if not args.no_configs:
-        print("[bench] 2D column (config 1) on the fp64 engine", file=sys.stderr, flush=True)
-        c1 = proto2d_c1(not args.no_cpu)
+        print("[bench] 2D 40x80 column on the fp64 engine", file=sys.stderr, flush=True)
+        c1 = column_2d(not args.no_cpu)
         print("[bench] mixed water + sand scene", file=sys.stderr, flush=True)
         mixed = mixed_scene(60, 10)
         if net is not None:
@@ -367,7 +367,7 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": (f"DP sand column 80x160x80 r=0.005 ({N} particles)" if args.config == "headline"
+        "config": {"workload": (f"DP sand column 80x160x80 r=0.005 ({N} particles)" if args.config == "column1m"
                                 else f"DP sand column 40x80x40 r=0.01 ({N} particles)")
                    + (", Jacobi to residual 1e-4 from the learned warm start" if resid else
                       f", {I} Jacobi iterations (no residual target: --no-residual)"),
@@ -387,7 +387,7 @@ def main():
         "roofline": roofline, "gather_kernels_roofline": gather, "kernels": kern,
         "stream_copy_GBs": round(copy_GBs, 1),
         "jacobi_to_residual": resid, "batched_ensemble": ens, "mixed_water_sand": mixed,
-        "vm_block_poke_to_residual": poke, "config1_2d": c1, "cpu_baseline": cpu,
+        "vm_block_poke_to_residual": poke, "column_2d": c1, "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 

@@ -9,7 +9,7 @@ Hencky with C = sqrt(2 Psi), Drucker-Prager return map in the loop, the 0.75h
 position correction, box walls with friction, XSPH, the final F update and the
 clipped position update).  The prototype is a script with hard-coded constants;
 here they are parameters (``prototype_params`` are its own values,
-``c1_params`` the 40x80 column at r = 0.01 m) and the state is explicit.
+``column40x80_params`` the 40x80 column at r = 0.01 m) and the state is explicit.
 
 Where the restatement fixes what the prototype leaves to a library, it says so:
 
@@ -56,7 +56,7 @@ def prototype_params():
     return make_params(0.008, 1e4, 0.3, 1000.0, 35.0, 2e-4, 8, [0.0, -9.8], 0.01, [1.0, 0.6])
 
 
-def c1_params():
+def column40x80_params():
     """The 40 x 80 column at r = 0.01 m: E = 3.5e5 Pa, nu = 0.3, rho = 2000, phi = 30 deg,
     g = -9.81, dt = 1/3000 s, 20 Jacobi iterations with Delta-lambda under-relaxed by 0.5,
     in a 2.0 x 1.0 m box."""
@@ -79,7 +79,7 @@ def prototype_state():
     return initial_state(column(int(0.15 / dx), int(0.30 / dx), dx, 0.425, dx, 0))
 
 
-def c1_state(seed=0):
+def column40x80_state(seed=0):
     """40 x 80 column at r = 0.01 m, centred in x on the 2.0 m floor, first row at y = r."""
     r = 0.01
     return initial_state(column(40, 80, r, 1.0 - 20 * r + 0.5 * r, r, seed))

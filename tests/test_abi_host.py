@@ -88,9 +88,9 @@ REF_COUNTS = {"Sand: column collapse": 12312, "Sand: two blocks collide": 15552,
 
 
 @pytest.mark.skipif(shutil.which("node") is None, reason="node not installed")
-@pytest.mark.parametrize("name", list(scenes.SCENES) + ["c2"])
+@pytest.mark.parametrize("name", list(scenes.SCENES) + ["column128k"])
 def test_js_scenes_bit_identical(name):
-    f = scenes.config_c2 if name == "c2" else scenes.SCENES[name]
+    f = scenes.sand_column_128k if name == "column128k" else scenes.SCENES[name]
     for seed in (0, 3):
         s = f(seed=seed)
         out = subprocess.check_output(["node", os.path.join(JS, "run.js"), "hash", name, str(seed)], text=True)
@@ -102,7 +102,7 @@ def test_js_scenes_bit_identical(name):
 
 
 def test_scene_generation_properties():
-    s = scenes.config_c2()
+    s = scenes.sand_column_128k()
     p = s["positions"].astype(np.float64)
     assert p.shape == (128000, 3)
     # lattice at r = 0.01 with jitter within +-0.05 r
@@ -114,8 +114,8 @@ def test_scene_generation_properties():
     tags = s["tags"]
     assert ((tags >= 0) & (tags < 0.49)).all()
     # seeded: same seed same scene, different seed different jitter
-    assert np.array_equal(scenes.config_c2(2)["positions"], s["positions"])
-    assert not np.array_equal(scenes.config_c2(5)["positions"], s["positions"])
+    assert np.array_equal(scenes.sand_column_128k(2)["positions"], s["positions"])
+    assert not np.array_equal(scenes.sand_column_128k(5)["positions"], s["positions"])
     u = scenes.rand_unit(1, 0, np.arange(200000))
     assert abs(u.mean() - 0.5) < 0.005 and u.min() >= 0 and u.max() < 1
 

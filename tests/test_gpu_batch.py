@@ -54,7 +54,7 @@ def test_batch_equals_separate_runs():
 
 
 def test_persistent_grid_equals_one_block_per_entry():
-    sl = scenes.config_c4(count=12)
+    sl = scenes.ensemble_64(count=12)
     a, b = _batch(sl), _batch(sl)
     n_entries = a.get_launch()[1]
     b.set_launch(64)
@@ -70,7 +70,7 @@ def test_persistent_grid_equals_one_block_per_entry():
 @pytest.mark.parametrize("mode", ["batch", "tolerance+warm"])
 def test_graph_replay_equals_eager(mode):
     if mode == "batch":
-        sl = scenes.config_c4(count=8)
+        sl = scenes.ensemble_64(count=8)
         a, b = _batch(sl), _batch(sl)
         iters, water, warm, tol = 20, False, None, 0.0
     else:
@@ -101,7 +101,7 @@ def test_graph_replay_equals_eager(mode):
 
 
 def test_autotune_keeps_state_and_results():
-    sl = scenes.config_c4(count=16)
+    sl = scenes.ensemble_64(count=16)
     a, b = _batch(sl), _batch(sl)
     a.step(10, 20, False)
     b.step(10, 20, False)

@@ -133,12 +133,12 @@ def one_substep_parity(sim, scene, oracle, label, iters=None, water=None):
     return compare(sim, variants, st64, label)
 
 
-def test_sort_bit_exact_c2(oracle):
+def test_sort_bit_exact_128k_column(oracle):
     """128k-particle DP column: cell starts and permutation equal the oracle and numpy's stable sort."""
-    scene = scenes.config_c2()
+    scene = scenes.sand_column_128k()
     sim = make_sim(scene)
     ref = oracle.initial_state(scene, sim.numCells)
-    print("sort c2: N", sim.N, "cells", sim.numCells, flush=True)
+    print("sort, 128k column: N", sim.N, "cells", sim.numCells, flush=True)
     sim.substep_partial(0, False, "SORT")
     oracle.substep(ref, sim.params, mats_record(scene), 0, False, stop="SORT")
     perm, cells = sim.read("PERM"), sim.read("CELLSTART")
@@ -216,7 +216,7 @@ def test_substep_parity_headline_column(oracle):
     """The benchmark's 1,024,000-particle DP column at full size, landed (300 substeps):
     the sort bit-exact and one substep (20 iterations) within the bar, against the f32
     oracle, its 1-ulp perturbations and the f64 oracle (OpenMP, a few seconds each)."""
-    scene = scenes.config_headline()
+    scene = scenes.sand_column_1m()
     sim = make_sim(scene)
     print("headline column: settling 300 substeps", flush=True)
     sim.step(300, 20, False)

@@ -33,10 +33,10 @@ def _compare(sim, st, what):
         assert v <= TOL, (what, k, v)
 
 
-@pytest.mark.parametrize("case,chunks", [("c1", [1, 4, 15]), ("prototype", [10, 40, 100])])
+@pytest.mark.parametrize("case,chunks", [("column40x80", [1, 4, 15]), ("prototype", [10, 40, 100])])
 def test_engine_matches_oracle(case, chunks):
-    if case == "c1":
-        p, x = host.c1_params(), host.c1_positions()
+    if case == "column40x80":
+        p, x = host.column40x80_params(), host.column40x80_positions()
     else:
         p, x = host.prototype_params(), host.prototype_positions()
     prm = _oracle_prm(p)
@@ -54,7 +54,7 @@ def test_engine_matches_oracle(case, chunks):
 
 
 def test_rest_lattice_and_momentum_on_device():
-    p = dict(host.c1_params(), domain=[4.0, 4.0], gravity=[0.0, 0.0])
+    p = dict(host.column40x80_params(), domain=[4.0, 4.0], gravity=[0.0, 0.0])
     ax = np.arange(12) * 0.01 + 1.5
     X, Y = np.meshgrid(ax, ax)
     lat = np.stack([X.ravel(), Y.ravel()], 1)
@@ -66,7 +66,7 @@ def test_rest_lattice_and_momentum_on_device():
     g = np.random.default_rng(8)
     x = host.column(16, 16, 0.01, 1.5, 1.5, seed=7) + g.normal(0, 0.002, (256, 2))
     v = g.normal(0, 0.05, (256, 2))
-    p = dict(host.c1_params(), domain=[4.0, 4.0])
+    p = dict(host.column40x80_params(), domain=[4.0, 4.0])
     sim = host.Prototype2D(x, p)
     sim.write("V", v)
     sim.step(1)
@@ -78,7 +78,7 @@ def test_rest_lattice_and_momentum_on_device():
 
 def test_neighbour_overflow_is_an_error():
     x = 0.5 + np.random.default_rng(1).uniform(-0.002, 0.002, (60, 2))  # 60 particles within 2h
-    sim = host.Prototype2D(x, host.c1_params())
+    sim = host.Prototype2D(x, host.column40x80_params())
     with pytest.raises(XpbiError, match="neighbours"):
         sim.step(1)
     sim.close()
@@ -87,7 +87,7 @@ def test_neighbour_overflow_is_an_error():
 def test_cooperative_grid_equals_single_workgroup():
     """The cooperative grid (one thread per particle, grid barriers) and one workgroup
     (workgroup barriers) run the same arithmetic: identical state, bit for bit."""
-    p, x = host.c1_params(), host.c1_positions()
+    p, x = host.column40x80_params(), host.column40x80_positions()
     out = {}
     for single in (False, True):
         sim = host.Prototype2D(x, p)

@@ -44,7 +44,7 @@ def test_svd2_against_lapack():
 
 
 def test_wendland_2d_normalisation_and_gradient():
-    prm = o.c1_params()
+    prm = o.column40x80_params()
     h = prm["h"]
     n = 801
     ax = np.linspace(-2 * h, 2 * h, n)
@@ -70,7 +70,7 @@ def test_wendland_2d_normalisation_and_gradient():
 
 
 def test_pairs_against_brute_force_and_kdtree():
-    x = host.c1_positions(seed=3)[:900]
+    x = host.column40x80_positions(seed=3)[:900]
     s = 0.02
     P, B = o.pairs(x, s)
     d = x[:, None, :] - x[None, :, :]
@@ -84,7 +84,7 @@ def test_pairs_against_brute_force_and_kdtree():
 
 
 def test_drucker_prager_cone_and_tip():
-    prm = o.c1_params()
+    prm = o.column40x80_params()
     g = np.random.default_rng(4)
     S = np.exp(g.normal(0, 0.02, (500, 2)))
     out = o.drucker_prager(prm, S)
@@ -107,7 +107,7 @@ def test_drucker_prager_cone_and_tip():
 
 
 def test_stvk_energy_and_derivative():
-    prm = o.c1_params()
+    prm = o.column40x80_params()
     psi, _ = o.stvk(prm, np.ones((1, 2)))
     assert psi[0] == 0.0
     g = np.random.default_rng(5)
@@ -135,7 +135,7 @@ def _lattice(n, h):
 
 
 def test_rest_lattice_stays_at_rest():
-    prm, st = _free(o.c1_params(), _lattice(12, 0.01))
+    prm, st = _free(o.column40x80_params(), _lattice(12, 0.01))
     prm["gravity"] = np.zeros(2)
     x0 = st["x"].copy()
     o.run(prm, st, 3)
@@ -143,17 +143,17 @@ def test_rest_lattice_stays_at_rest():
 
 
 def test_free_fall_without_neighbours():
-    prm, st = _free(o.c1_params(), np.stack([np.arange(50) * 0.1, np.zeros(50)], 1))
+    prm, st = _free(o.column40x80_params(), np.stack([np.arange(50) * 0.1, np.zeros(50)], 1))
     o.run(prm, st, 10)
     assert np.allclose(st["v"][:, 1], 10 * prm["dt"] * prm["gravity"][1], rtol=1e-13)
     assert np.array_equal(st["F"], np.tile(np.eye(2), (50, 1, 1)))
 
 
-@pytest.mark.parametrize("params", ["prototype", "c1"])
+@pytest.mark.parametrize("params", ["prototype", "column40x80"])
 def test_internal_impulses_conserve_momentum(params):
     """Away from the walls, with V^n uniform (F = I), the constraint gather, the
     position correction and XSPH move no net momentum: sum v changes by N g dt."""
-    prm = o.prototype_params() if params == "prototype" else o.c1_params()
+    prm = o.prototype_params() if params == "prototype" else o.column40x80_params()
     h = prm["h"]
     x = host.column(16, 16, h, 0.0, 0.0, seed=7)
     g = np.random.default_rng(8)
@@ -168,6 +168,6 @@ def test_internal_impulses_conserve_momentum(params):
 
 
 def test_host_and_oracle_columns_agree():
-    assert np.array_equal(host.c1_positions(), o.c1_state()["x"])
+    assert np.array_equal(host.column40x80_positions(), o.column40x80_state()["x"])
     assert np.array_equal(host.prototype_positions(), o.prototype_state()["x"])
-    assert len(host.prototype_positions()) == 666 and len(host.c1_positions()) == 3200
+    assert len(host.prototype_positions()) == 666 and len(host.column40x80_positions()) == 3200

@@ -1,6 +1,6 @@
 'use strict';
 // Command-line driver of the JavaScript host.
-//   node run.js hash  <scene name | c2 | headline> <seed>   sha256 of the generated scene arrays (no GPU)
+//   node run.js hash  <scene name | column128k | column1m> <seed>   sha256 of the generated scene arrays (no GPU)
 //   node run.js frames <scene name> <seed> <frames> <out.bin>  run frames on HIP device 0 and write
 //                                                         POS (float32) followed by ID (uint32)
 //   node run.js batch <scene name>,<scene name>,... <seed> <substeps> <out.bin>   a batch of the scenes:
@@ -12,8 +12,8 @@ const fs = require('fs');
 const scenes = require('./scenes');
 
 function makeScene(name, seed) {
-  if (name === 'c2') return scenes.configC2(seed);
-  if (name === 'headline') return scenes.configHeadline(seed);
+  if (name === 'column128k') return scenes.sandColumn128k(seed);
+  if (name === 'column1m') return scenes.sandColumn1m(seed);
   const f = scenes.SCENES[name];
   if (!f) throw new Error('unknown scene ' + name);
   return f(seed);

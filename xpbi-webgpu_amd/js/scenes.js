@@ -209,10 +209,10 @@ function sandColumn(nx, ny, nz, r, domain, seed, phi, dt, iterations, jitter) {
   sampleBox(out, mn, mx, r, 0, rng, jitter === undefined ? 0.1 : jitter);
   return finish(s, out, [SAND(phi, 0, 3.5e5, 0.3, 2000)], seed, false, null, `DP sand column ${nx}x${ny}x${nz}`);
 }
-const configC2 = (seed = 2) => sandColumn(40, 80, 40, 0.01, [2, 1, 2], seed);
-const configHeadline = (seed = 1) => sandColumn(80, 160, 80, 0.005, [1.5, 1, 1.5], seed);
+const sandColumn128k = (seed = 2) => sandColumn(40, 80, 40, 0.01, [2, 1, 2], seed);
+const sandColumn1m = (seed = 1) => sandColumn(80, 160, 80, 0.005, [1.5, 1, 1.5], seed);
 
 module.exports = {
   SCENES, sampleBox, sampleSphere, base, dpAlpha, SAND, SNOW_NACC, PLASTICINE, GOO, SNOW_CLASSIC, WATER, ELASTIC,
-  sandColumn, configC2, configHeadline, lowbias32, randU32, randUnit, Rng,
+  sandColumn, sandColumn128k, sandColumn1m, lowbias32, randU32, randUnit, Rng,
 };

@@ -7,7 +7,7 @@ the state by substeps of dt.  ``Prototype2D`` holds that state on the device
 attributes ``x``, ``v``, ``F`` and ``lam`` read it back under the script's names.
 
 ``prototype_params()`` / ``prototype_positions()`` are the script's own constants
-and initial column (666 particles); ``c1_params()`` / ``c1_positions()`` the 40 x 80
+and initial column (666 particles); ``column40x80_params()`` / ``column40x80_positions()`` the 40 x 80
 column at r = 0.01 m (E = 3.5e5 Pa, rho = 2000, phi = 30 deg, dt = 1/3000 s,
 20 iterations, Delta-lambda under-relaxed by 0.5).
 """
@@ -31,7 +31,7 @@ def prototype_params():
     return params(0.008, 1e4, 0.3, 1000.0, 35.0, 2e-4, 8, [0.0, -9.8], 0.01, [1.0, 0.6])
 
 
-def c1_params():
+def column40x80_params():
     return params(0.01, 3.5e5, 0.3, 2000.0, 30.0, 1.0 / 3000.0, 20, [0.0, -9.81], 0.01, [2.0, 1.0], omega=0.5)
 
 
@@ -48,7 +48,7 @@ def prototype_positions():
     return column(int(0.15 / dx), int(0.30 / dx), dx, 0.425, dx, 0)
 
 
-def c1_positions(seed=0):
+def column40x80_positions(seed=0):
     r = 0.01
     return column(40, 80, r, 1.0 - 20 * r + 0.5 * r, r, seed)
 

@@ -281,17 +281,17 @@ def sand_column(nx, ny, nz, r, domain, seed, phi=30.0, dt=1.0 / 6000.0, iteratio
     return _finish(s, out, [DP_SAND_STIFF(phi)], seed, False, info=f"DP sand column {nx}x{ny}x{nz}")
 
 
-def config_c2(seed=2):
+def sand_column_128k(seed=2):
     """3D DP sand column 40x80x40 at r = 0.01 (128,000 particles), dt = 1/6000, 20 iterations."""
     return sand_column(40, 80, 40, 0.01, [2.0, 1.0, 2.0], seed)
 
 
-def config_headline(seed=1):
+def sand_column_1m(seed=1):
     """1M DP sand: 80x160x80 at r = 0.005 (1,024,000 particles), dt = 1/6000, 20 iterations."""
     return sand_column(80, 160, 80, 0.005, [1.5, 1.0, 1.5], seed)
 
 
-def config_c3(seed=3, dt=1.0 / 6000.0, iterations=10):
+def dam_breach_1m(seed=3, dt=1.0 / 6000.0, iterations=10):
     """Dam breach: 60x100x100 PBF water next to a 50x80x100 DP sand wall (phi 35), r = 0.005."""
     r = 0.005
     s = _solver(r, dt, iterations, pbfEps=100.0, sCorrK=0.1)
@@ -318,12 +318,12 @@ def vm_poke(n, r, seed, dt=1.0 / 6000.0, iterations=20, sigY=5e3):
                    info=f"von Mises block {n}^3 + poke")
 
 
-def config_c5(seed=5, dt=1.0 / 6000.0, iterations=20):
+def vm_poke_1m(seed=5, dt=1.0 / 6000.0, iterations=20):
     """von Mises block 100^3 at r = 0.005 (sigY 5e3, E 2e5, nu 0.3) with a 0.05 m sphere at 1 m/s."""
     return vm_poke(100, 0.005, seed, dt, iterations)
 
 
-def config_c4(seed=4, count=64, dt=1.0 / 6000.0, iterations=20, r=0.01):
+def ensemble_64(seed=4, count=64, dt=1.0 / 6000.0, iterations=20, r=0.01):
     """Batched ensemble: `count` independent scenes with particle counts log-uniform in
     2k-60k (seeded), each a jittered block dropped on its own floor, its material drawn
     uniformly from Drucker-Prager (phi 20-40 deg), NACC (xi 0-10, beta 0.3), von Mises

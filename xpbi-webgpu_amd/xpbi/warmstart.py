@@ -74,7 +74,7 @@ def training_columns():
     fraction): DP sand columns (phi 30) collapsing on the floor at the two resolutions
     of the 128k and 1M configurations, and a von Mises block poked by a moving sphere
     (125k particles at r = 0.01, unlike the 1M poke benchmark), each with its own seed."""
-    return [("DP sand column 40x80x40 r=0.01 (128,000)", scenes.config_c2(), 300, 40, 0.25),
+    return [("DP sand column 40x80x40 r=0.01 (128,000)", scenes.sand_column_128k(), 300, 40, 0.25),
             ("DP sand column 40x80x40 r=0.005 (128,000)",
              scenes.sand_column(40, 80, 40, 0.005, [0.75, 0.5, 0.75], seed=7), 420, 40, 0.25),
             ("von Mises block 50^3 r=0.01 + poke (125,000)", scenes.vm_poke(50, 0.01, seed=9), 440, 40, 0.25)]

@@ -238,8 +238,6 @@ int xpbi_create_batch(const xpbi_scene_desc* descs, int count, xpbi_sim** out) {
   if (e != hipSuccess) { int r = hip_fail(nullptr, e, "hipSetDevice"); delete h; return r; }
   e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { int r = hip_fail(nullptr, e, "hipStreamCreate"); delete h; return r; }
-  if (hipDeviceGetAttribute(&s->cus, hipDeviceAttributeMultiprocessorCount, s->dev) != hipSuccess || s->cus < 1)
-    s->cus = 256;
 
   const u32 N = s->N, C = s->C;
   const u32 gN = s->nb;

@@ -83,7 +83,6 @@ struct Scene {
 
 struct Sim {
   int dev = 0;
-  int cus = 256;  // compute units of the device
   hipStream_t stream = nullptr;
   u32 N = 0, C = 0;  // all scenes
   std::vector<Scene> sc;

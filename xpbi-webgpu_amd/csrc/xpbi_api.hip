@@ -5,6 +5,7 @@
 // (web/js/main.js:289-313).
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -93,6 +94,7 @@ void build_kp(Scene& sc) {
   k.invH = 1.0f / h;
   k.halfInvH = 0.5f / h;
   k.gradC = -5.0f * k.sigW / (h * h);
+  k.invW0 = 1.0f / std::max(k.w0, 1e-12f);
   k.cellBase = sc.cellBase;
   k.pBase = sc.pBase;
   k.pEnd = sc.pBase + sc.N;

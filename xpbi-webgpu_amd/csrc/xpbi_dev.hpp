@@ -55,6 +55,7 @@ struct KP {
   float invH;    // 1 / h
   float halfInvH;  // 0.5 / h
   float gradC;   // -5 sigma / h^2: grad W = dxv * gradC * (1 - q/2)^3
+  float invW0;   // 1 / max(w0, 1e-12): the s_corr ratio W / W(0.2 support) as a product
   // the scene's place in a batch: its cells are cellBase .. cellBase + C - 1 of
   // the shared cell table, its particles pBase .. pEnd - 1 of the shared arrays
   u32 cellBase, pBase, pEnd;
@@ -212,13 +213,13 @@ __device__ __forceinline__ float kernelPoly6(const KP& P, float r) {
   float x = H * H - r * r;
   return P.poly6C * x * x * x;
 }
-// common.js:88-94
-__device__ __forceinline__ f3 kernelSpikyGrad(const KP& P, f3 dxv) {
+// common.js:88-94 with the pair distance r = |dxv| already formed and the
+// division by r as a product with the hardware reciprocal (1 ulp)
+__device__ __forceinline__ f3 kernelSpikyGradR(const KP& P, f3 dxv, float r) {
   float H = P.support;
-  float r = len(dxv);
   if (r >= H || r < 1e-9f) return splat3(0.0f);
   float t = H - r;
-  return dxv * (P.spikyC * t * t / r);
+  return dxv * (P.spikyC * t * t * __builtin_amdgcn_rcpf(r));
 }
 
 // ---------------------------------------------------------------- grid hash

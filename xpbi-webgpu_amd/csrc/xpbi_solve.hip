@@ -270,8 +270,9 @@ __global__ __launch_bounds__(256) void k_pbfA(const KP* __restrict__ kps, WorkTa
         if (dot(dxv, dxv) < P.supp2) {
           const u32 kb = matOfTag(xb.w);
           float mb = T.mass[kb];
-          rho = rho + mb * kernelPoly6(P, len(dxv));
-          f3 g = kernelSpikyGrad(P, dxv) * mRho[kb];
+          const float r = len(dxv);
+          rho = rho + mb * kernelPoly6(P, r);
+          f3 g = kernelSpikyGradR(P, dxv, r) * mRho[kb];
           gradSum = gradSum + g;
           grad2 = grad2 + dot(g, g);
         }
@@ -313,11 +314,12 @@ __global__ __launch_bounds__(256) void k_pbfB(const KP* __restrict__ kps, WorkTa
           if (dot(dxv, dxv) < P.supp2) {
             u32 wk = bWater ? mb : mi;
             float lb = r.l;
-            float w = kernelPoly6(P, len(dxv));
-            float ratio = w / wmax(P.w0, 1e-12f);
+            const float r = len(dxv);
+            float w = kernelPoly6(P, r);
+            float ratio = w * P.invW0;
             float r2 = ratio * ratio;
             float sCorr = -P.sCorrK * (r2 * r2);
-            dp = dp + (li + lb + sCorr) * kernelSpikyGrad(P, dxv) * T.mRho[wk][wk];
+            dp = dp + (li + lb + sCorr) * kernelSpikyGradR(P, dxv, r) * T.mRho[wk][wk];
           }
         }
       }

@@ -104,6 +104,32 @@ def test_wide_wall_offsets_past_16_bits(oracle):
     sim.close()
 
 
+def test_list_capacity_changes_nothing_dam_breach_1m():
+    """The 1M dam breach (600k PBF water, 400k DP sand) at full size: 2 substeps from
+    the state after 60, once through the list (256 slots) and once walking every row
+    (0 slots), give the same state bit for bit.  A whole-state property: this scene
+    drives its water to the velocity clamp within a substep (the CPU oracle does the
+    same), so a 1-substep tolerance against the oracle would pin little here."""
+    from xpbi import scenes as sc_mod
+    scene = sc_mod.dam_breach_1m()
+    sim = make_sim(scene)
+    print("dam breach 1M: settling 60 substeps", flush=True)
+    sim.step(60, scene["iterations"], True)
+    snap = {k: sim.read(k) for k in ("POS", "VEL", "F", "AUX", "ID")}
+    out = {}
+    for slots in (256, 0):
+        for k, v in snap.items():
+            sim.write(k, v)
+        sim.set_neighbour_slots(slots)
+        sim.step(2, scene["iterations"], True)
+        out[slots] = {k: sim.read(k) for k in ("POS", "VEL", "F", "AUX")}
+        print(f"dam breach 1M: {slots} slots done", flush=True)
+    assert np.abs(out[256]["POS"] - snap["POS"]).max() > 0, "the state did not move"
+    for k in out[256]:
+        assert np.array_equal(out[0][k], out[256][k]), f"dam breach 1M: {k} differs with 0 slots"
+    sim.close()
+
+
 @pytest.mark.parametrize("name", ["sand column", "Dam breach: water + sand"])
 def test_list_capacity_changes_nothing(name):
     """8 substeps with the default 256 slots, with 16 (many particles past the list,

@@ -110,8 +110,7 @@ def test_list_capacity_changes_nothing_dam_breach_1m():
     (0 slots), give the same state bit for bit.  A whole-state property: this scene
     drives its water to the velocity clamp within a substep (the CPU oracle does the
     same), so a 1-substep tolerance against the oracle would pin little here."""
-    from xpbi import scenes as sc_mod
-    scene = sc_mod.dam_breach_1m()
+    scene = scenes.dam_breach_1m()
     sim = make_sim(scene)
     print("dam breach 1M: settling 60 substeps", flush=True)
     sim.step(60, scene["iterations"], True)

@@ -351,7 +351,7 @@ def main():
     ens = None
     if not args.no_ensemble:
         print("[bench] 64-scene batched ensemble", file=sys.stderr, flush=True)
-        ens = ensemble(I, 60, 20)
+        ens = ensemble(I, 300, 20)
     mixed = poke = c1 = None
     if not args.no_configs:
         print("[bench] 2D 40x80 column on the fp64 engine", file=sys.stderr, flush=True)

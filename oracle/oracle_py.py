@@ -84,6 +84,26 @@ def lib():
     return _lib
 
 
+class use_library:
+    """Context manager: the functions of this module call the oracle library at
+    `path` (a build of a modified copy of the sources) instead of liboracle.so."""
+
+    def __init__(self, path):
+        self.path = path
+
+    def __enter__(self):
+        global _lib, LIB
+        self.saved = (_lib, LIB)
+        _lib, LIB = None, self.path
+        lib()
+        return self
+
+    def __exit__(self, *exc):
+        global _lib, LIB
+        _lib, LIB = self.saved
+        return False
+
+
 def _p(a):
     return a.ctypes.data
 

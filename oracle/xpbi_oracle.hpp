@@ -1106,7 +1106,9 @@ template <class T> struct Sim {
           const T* row = feat + 16 * (size_t)i;
           double y[2];
           mlpForward(wsParams, row, y);
-          lam0[i] = row[12] == T(0) ? T(0) : wclamp(T(float(y[0])), T(0), T(2)) * row[12];
+          // a NaN output gives lambda0 = 0 for the particle (plain Jacobi from zero)
+          const T y0 = T(float(y[0]));
+          lam0[i] = (row[12] == T(0) || y0 != y0) ? T(0) : wclamp(y0, T(0), T(2)) * row[12];
           hbHint[i] = T(float(y[1]));
         }
         solveA(-1, lam0.data());

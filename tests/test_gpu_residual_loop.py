@@ -84,3 +84,36 @@ def test_stop_with_water_and_cap():
     assert a.last_iterations() == (5, -1.0)
     a.close()
     b.close()
+
+
+def test_learned_residual_loop_on_a_persistent_grid():
+    """The learned warm start presets the residual's denominator (sum C^2 at lambda = 0,
+    reduced per block by the features pass).  On a persistent grid each thread walks
+    several 256-particle runs, so its partial must accumulate over them: the loop then
+    stops after the same iterations at the same residual (up to the order of the block
+    sums) as with one block per run."""
+    from test_gpu_warmstart import _net
+    scene = scenes.sand_column(24, 48, 24, 0.01, [0.8, 0.6, 0.8], seed=11, iterations=20)
+    a = _sim(scene, 260)
+    snap = _state(a)
+    b = _sim(scene, 0)
+    entries = a.get_launch()[1]
+    b.set_launch(8)
+    assert b.get_launch() == (8, entries) and entries >= 32, "the persistent grid walks several runs per block"
+    net = _net(3)
+    for sim in (a, b):
+        for k, v in snap.items():
+            sim.write(k, v)
+        sim.set_warmstart(net)
+        sim.set_tolerance(1e-4, 4)
+    for step in range(3):
+        a.substep(200, False)
+        b.substep(200, False)
+        (na, ra), (nb, rb) = a.last_iterations(), b.last_iterations()
+        print(f"substep {step}: one block per run {na} iterations r {ra:.6e} | persistent {nb} r {rb:.6e}",
+              flush=True)
+        assert na == nb and 0 < na < 200
+        assert abs(ra - rb) <= 1e-5 * ra
+        assert ra <= 1e-4
+    a.close()
+    b.close()

@@ -265,7 +265,8 @@ __global__ __launch_bounds__(256) void k_mlp_pack(const float* __restrict__ w, M
 
 // ---------------------------------------------------------------- warm start in the substep
 // lambda0 = clamp(y0, 0, 2) d0 per particle (d0 = feature row slot 12, zero for
-// particles without an active constraint); the unscaled Sp / selfDv written by
+// particles without an active constraint; fmaxf maps a NaN y0 to 0, so such a
+// particle starts plain Jacobi from zero); the unscaled Sp / selfDv written by
 // the features kernel are scaled by lambda0, and y1 is the HB bracket position.
 // One chunk of 32 kMlpWaves particles per block: a grid-strided variant that
 // staged the weights once per block measured slower (85 against 60 us at 1M

@@ -500,7 +500,7 @@ __global__ __launch_bounds__(256) void k_features(const KP* __restrict__ kps, Wo
         row[15] = -c.C - g * P.dt * c.cdot;
         stm(Sp, i, c.GL);
         selfDv[i] = make_float4(c.sgrad.x, c.sgrad.y, c.sgrad.z, 0.0f);
-        c2 = c.C * c.C;
+        c2 += c.C * c.C;  // a persistent grid's thread walks several runs
         wrote = true;
       }
     }

@@ -39,6 +39,30 @@
 //   pbfA/pbfB                 water below rho0 (checked with the oracle's own
 //                             poly6) gets lambda_pbf = 0 and, without s_corr,
 //                             stays at rest
+//   computeL + velocity grad (test_oracle_pins.py) a linear velocity field v = G x gives the
+//                             corrected gradient G P, P the projection onto the right
+//                             singular vectors of A (numpy/LAPACK) above 0.2, including a
+//                             particle with two directions dropped
+//   solveA + solveB           the first damped XPBD step from lambda = 0 for one active
+//                             constraint equals -(C + gamma dC/dv.v) / ((1 + gamma)
+//                             sum |dC/dv_k|^2 / (m_k dt^2) + 1 / (V0 dt^2)) with dC/dv_k
+//                             from central differences of C, and every velocity moves by
+//                             dlambda dC/dv_k / (m_k dt^2)
+//   applyBoundary             floor: v_n = (bound - x) / dt, tangential speed cut by
+//                             min(mu |dv_n|, |v_t|); sphere: outward relative speed
+//                             min((R - d) / dt, 0.6), tangential motion kept
+//   contacts                  an isolated pair below 0.75 h: push (0.75 h - d) shared
+//                             as m_b / (m_a + m_b), momentum kept
+//   pbfA / pbfB               lambda_j (|J_j|^2 + eps) = -C_j with J the Jacobian of the
+//                             density constraints (eps 100 and 0); correction J^T lambda;
+//                             s_corr(q H) = -k ((1 - q^2) / 0.96)^12 at four distances
+//   xsph                      isolated pairs: c V_b (v_b - v_i) W(r), 10 c for water, no
+//                             solid-water smoothing
+//   planted mistakes          test_oracle_mutations.py: 15 plausible slips (pinv
+//                             tolerance, V_b in A, alpha~, (1 + gamma), the damping sign,
+//                             selfDv, gather sign, contact weight, friction, wall speed,
+//                             sphere cap, PBF eps, PBF neighbour lambda, s_corr exponent,
+//                             XSPH sign) each fail one of these pins
 // Parity of this oracle with the reference WGSL *as executed by a WebGPU stack*
 // is unpinned: no WebGPU implementation exists in this environment.
 #include "xpbi_oracle.hpp"

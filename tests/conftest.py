@@ -15,6 +15,12 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def oracle():
+    """The CPU oracle (oracle/).  XPBI_ORACLE_LIB names another build of it: the
+    mutation test runs the oracle's pins against copies with planted mistakes."""
     import oracle_py
-    oracle_py.build()
+    alt = os.environ.get("XPBI_ORACLE_LIB")
+    if alt:
+        oracle_py.use_library(alt).__enter__()
+    else:
+        oracle_py.build()
     return oracle_py

@@ -177,7 +177,7 @@ int xpbi_set_launch(xpbi_sim* sim, int blocks);
 /* Slots of the support-neighbour list (no reference counterpart: the reference
  * walks the 9 cell rows in every sweep, web/js/shaders/simkernels.js:59-74, and
  * this library records the neighbours once per substep instead).  Default 256, at
- * 2 bytes per slot per particle; 0..1024.  A particle with more neighbours walks
+ * 2 bytes per slot per particle; 0..1024, rounded up to a multiple of 16.  A particle with more neighbours walks
  * its cell rows, and slots = 0 walks rows everywhere and frees the list: the
  * results are the same bit for bit, only the speed and the memory differ. */
 int xpbi_set_neighbour_slots(xpbi_sim* sim, int slots);

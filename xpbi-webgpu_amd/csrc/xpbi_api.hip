@@ -815,6 +815,7 @@ int xpbi_set_neighbour_slots(xpbi_sim* h, int slots) {
   if (!h) return XPBI_ERR_ARG;
   Sim* s = &h->s;
   if (slots < 0 || slots > (int)Sim::kMaxNbrSlots) return fail(s, XPBI_ERR_ARG, "slots must be in 0..1024");
+  slots = (slots + 15) & ~15;  // the list stores slots in groups of 16
   if ((u32)slots == s->nbrK) return XPBI_OK;
   int r = xpbi_sync(h);
   if (r) return r;

@@ -3,6 +3,11 @@
 #pragma once
 #include "xpbi_internal.hpp"
 
+// Multiply-adds are fused only inside one source expression in this file, never
+// across statements: a sweep's per-neighbour code rounds the same wherever it is
+// inlined.
+#pragma clang fp contract(on)
+
 namespace xpbi {
 
 constexpr u32 kRowCap = 384u;  // simkernels.js:73
@@ -80,72 +85,176 @@ struct NLc {
   const u16* __restrict__ lst;
   u32 K, N;
 };
+// The list holds a particle's slots in groups of 16, interleaved for the four
+// lanes of a quad (see forSupport): group g of particle i is the 32 bytes at
+// lst + (g N + i) 16, and slot s = 16 g + 4 t + q (q = s mod 4) sits at position
+// 4 q + t in it.  Lane q then reads its next four slots (q, q + 4, q + 8,
+// q + 12 of the group) as one aligned 8-byte word, and a wave reads 16
+// consecutive particles' groups as 512 contiguous bytes.  K is a multiple of 16.
+__device__ __forceinline__ size_t slotAt(u32 s, u32 i, u32 N) {
+  return ((size_t)(s >> 4) * N + i) * 16u + ((s & 3u) << 2) + ((s >> 2) & 3u);
+}
 
-#ifndef XPBI_GATHER_BATCH
-#define XPBI_GATHER_BATCH 8
-#endif
-// the constraint gather of solveB carries 17 floats per neighbour: its own depth
-#ifndef XPBI_GATHER_BATCH_B
-#define XPBI_GATHER_BATCH_B 6
-#endif
-constexpr int kBatch = XPBI_GATHER_BATCH;
-constexpr int kBatchB = XPBI_GATHER_BATCH_B;
+// |d|^2 with its rounding fixed by explicit fused multiply-adds: computeL's
+// acceptance test and every walk below evaluate it alike
+__device__ __forceinline__ float dist2(f3 d) { return fmaf(d.z, d.z, fmaf(d.y, d.y, d.x * d.x)); }
 
-// Support-neighbour walk with the gathers issued in batches of kBatch
-// independent loads (and the next batch's indices prefetched) so that several
-// cache misses are in flight per lane.  load(b) gathers the neighbour's record,
-// whose member p is (x_b, tag_b); use(b, rec, dxv = x_i - x_b, |dxv|^2)
-// consumes it, in list order.
-template <int kB = kBatch, class Load, class Use>
-__device__ __forceinline__ void forSupportG(const KP& P, const u32* __restrict__ cs, const NLc& nl, u32 i, f3 xi,
-                                            Load&& load, Use&& use) {
-  const uint2 hd = nl.head[i];
-  const u32 n = hd.y;
-  if (n <= nl.K) {
-    // raw entries of the next batch (past the count: the particle's own offset,
-    // so a batch's tail gathers i itself); the base is added when the batch is
-    // used, so that no arithmetic on a loaded entry sits inside the load's
-    // predicate (which would wait on every load in flight).  Slot kk's row
-    // nl.lst + kk N is uniform: the entry load is a scalar base plus 2 i.  (The
-    // predicate stays: unpredicated entry loads, over a list padded with the own
-    // offset, made the compiler wait for a whole batch of gathers before its
-    // first use and measured 13 % slower.)
-    const u32 self = i - hd.x;
-    u32 en[kB];
-#pragma unroll
-    for (int j = 0; j < kB; j++) en[j] = ((u32)j < n) ? (u32)gat(nl.lst + (size_t)j * nl.N, i) : self;
-    for (u32 k = 0; k < n; k += kB) {
-      u32 b[kB];
-      decltype(load(0u)) d[kB];
-#pragma unroll
-      for (int j = 0; j < kB; j++) {
-        b[j] = hd.x + en[j];
-        d[j] = load(b[j]);
+// A walk over the particle's cell rows that yields its support neighbours one at
+// a time, in the reference's candidate order: the fallback of a particle whose
+// list overflowed.  next() returns the next neighbour of lane q (neighbour
+// number c belongs to lane c mod 4), or ~0u once the rows are exhausted.
+struct RowCursor {
+  i3 cc;
+  u32 x0, x1;
+  int r;         // current row 0..8 (dz = r / 3 - 1, dy = r % 3 - 1); 9 = done
+  u32 b, end;    // remaining candidates of the current row
+  u32 cnt;       // neighbours found so far
+  __device__ __forceinline__ void init(const KP& P, f3 xi) {
+    cc = cellCoord(P, xi);
+    x0 = (u32)max(cc.x - 1, 0);
+    x1 = (u32)min(cc.x + 1, P.gx - 1);
+    r = -1;
+    b = end = 0;
+    cnt = 0;
+  }
+  __device__ __forceinline__ u32 next(const KP& P, const u32* __restrict__ cs, const float4* __restrict__ pos, u32 i,
+                                      f3 xi, u32 q) {
+#pragma unroll 1
+    for (;;) {
+#pragma unroll 1
+      while (b >= end) {
+        if (r >= 8) return ~0u;
+        r++;
+        const int z = cc.z + r / 3 - 1, y = cc.y + r % 3 - 1;
+        if (z < 0 || z >= P.gz || y < 0 || y >= P.gy) continue;
+        const u32 row = P.cellBase + (u32)y * (u32)P.gx + (u32)z * P.gxy;
+        b = cs[row + x0];
+        end = min(cs[row + x1 + 1u], b + kRowCap);
       }
-#pragma unroll
-      for (int j = 0; j < kB; j++) {
-        u32 kk = k + kB + (u32)j;
-        en[j] = (kk < n) ? (u32)gat(nl.lst + (size_t)kk * nl.N, i) : self;
-      }
-#pragma unroll
-      for (int j = 0; j < kB; j++) {
-        if (k + (u32)j < n) {
-          const f3 dd = xi - xyz(d[j].p);
-          use(b[j], d[j], dd, dot(dd, dd));
-        }
+      const u32 c = b++;
+      if (c == i) continue;
+      if (dist2(xi - xyz(pos[c])) < P.supp2) {
+        const bool mine = (cnt & 3u) == q;
+        cnt++;
+        if (mine) return c;
       }
     }
-  } else {
-    forRows(P, cs, cellCoord(P, xi), [&](u32 b) {
-      if (b != i) {
-        const auto rec = load(b);
-        const f3 d = xi - xyz(rec.p);
-        const float r2 = dot(d, d);
-        if (r2 < P.supp2) use(b, rec, d, r2);
+  }
+};
+
+// Support-neighbour walk of particle i by the four lanes of a quad: lane q
+// (0..3) takes the neighbours numbered q, q + 4, q + 8, ..., so that one
+// wave-wide gather reads four consecutive neighbours (mostly adjacent records of
+// one cell row) of each of 16 consecutive particles, and each lane reads its
+// list entries as one 8-byte word per four neighbours.  load(b) gathers a
+// neighbour's record, whose member p is (x_b, tag_b); use(b, rec, dxv = x_i -
+// x_b, |dxv|^2) consumes it, in the lane's order; the four records of a word are
+// loaded together.  A particle whose list overflowed takes its neighbours from a
+// RowCursor instead: the same neighbours in the same lanes and order, through
+// the same single copy of the loads and of use().
+template <class Load, class Use>
+__device__ __forceinline__ void forSupport(const KP& P, const u32* __restrict__ cs, const float4* __restrict__ pos,
+                                           const NLc& nl, u32 i, f3 xi, u32 q, Load&& load, Use&& use) {
+  const uint2 hd = nl.head[i];
+  const u32 n = hd.y;
+  const bool rows = n > nl.K;
+  // list: the lane's word of group g (slots 16 g + 4 j + q, j = 0..3) is 8-byte
+  // word (g N + i) 4 + q; the next group's word is prefetched
+  const uint64_t* __restrict__ lw = (const uint64_t*)nl.lst + ((size_t)i * 4u + q);
+  const size_t gstep = (size_t)nl.N * 4u;  // 8-byte words per group
+  u32 s0 = q;                              // the lane's first slot of the group
+  uint64_t w = (!rows && s0 < n) ? lw[0] : 0ull;
+  RowCursor rc;
+  if (rows) rc.init(P, xi);
+#pragma unroll 1
+  for (u32 g = 0;; g++) {
+    u32 b[4];
+    bool v[4];
+    if (!rows) {
+      if (s0 >= n) break;
+      const uint64_t wn = (s0 + 16u < n) ? lw[(size_t)(g + 1u) * gstep] : 0ull;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        v[j] = s0 + 4u * (u32)j < n;
+        b[j] = v[j] ? hd.x + (u32)((w >> (16 * j)) & 0xffffu) : i;
       }
-    });
+      w = wn;
+      s0 += 16u;
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const u32 c = rc.next(P, cs, pos, i, xi, q);
+        v[j] = c != ~0u;
+        b[j] = v[j] ? c : i;
+      }
+      if (!v[0]) break;
+    }
+    decltype(load(0u)) d[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) d[j] = load(b[j]);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      if (v[j]) {
+        const f3 dd = xi - xyz(d[j].p);
+        use(b[j], d[j], dd, dist2(dd));
+      }
+    }
+    if (!v[3]) break;
   }
 }
+
+// sum over the four lanes of a quad, ((l0 + l1) + (l2 + l3)) in every lane (the
+// two xor steps add the same pairs in every lane: f32 addition commutes)
+__device__ __forceinline__ float quadSum(float x) {
+  x += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xB1, 0xF, 0xF, false));  // quad_perm 1,0,3,2
+  x += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x4E, 0xF, 0xF, false));  // quad_perm 2,3,0,1
+  return x;
+}
+__device__ __forceinline__ f3 quadSum(f3 v) { return f3{quadSum(v.x), quadSum(v.y), quadSum(v.z)}; }
+
+// The per-particle sums of a block's run of up to 256 particles, in four passes
+// of 64: in pass p the quad of lanes 4 g .. 4 g + 3 gathers particle
+// first + 64 p + g (gather(i, q, acc) adds lane q's share), the quad's shares
+// are summed, and the quad's first lane parks the sum in LDS (lds: Acc::kF rows
+// of 256 floats, one column per particle of the run).  Returns the sum of
+// particle first + threadIdx.x, which the thread then finishes.  Acc provides
+// zero(), quadSum(), put(lds, slot) and get(lds, slot).  Contains block
+// barriers: every thread of the block calls it.
+template <class Acc, class Gather>
+__device__ __forceinline__ Acc runPasses(float (*lds)[256], u32 first, u32 pEnd, Gather&& gather) {
+  const u32 q = threadIdx.x & 3u, g = threadIdx.x >> 2;
+#pragma unroll 1
+  for (u32 pass = 0; pass < 4u; pass++) {
+    const u32 base = first + 64u * pass;
+    if (base >= pEnd) break;
+    const u32 i = base + g;
+    Acc a = Acc::zero();
+    if (i < pEnd) gather(i, q, a);
+    a.quadSum();
+    if (q == 0u) a.put(lds, 64u * pass + g);
+  }
+  __syncthreads();
+  const Acc own = Acc::get(lds, threadIdx.x);
+  __syncthreads();  // the next run of a persistent block rewrites the slots
+  return own;
+}
+
+__device__ __forceinline__ void putF3(float (*lds)[256], int row, u32 slot, f3 v) {
+  lds[row][slot] = v.x;
+  lds[row + 1][slot] = v.y;
+  lds[row + 2][slot] = v.z;
+}
+__device__ __forceinline__ f3 getF3(float (*lds)[256], int row, u32 slot) {
+  return f3{lds[row][slot], lds[row + 1][slot], lds[row + 2][slot]};
+}
+struct F3Acc {
+  static constexpr int kF = 3;
+  f3 v;
+  __device__ __forceinline__ static F3Acc zero() { return F3Acc{splat3(0.0f)}; }
+  __device__ __forceinline__ void quadSum() { v = xpbi::quadSum(v); }
+  __device__ __forceinline__ void put(float (*lds)[256], u32 slot) const { putF3(lds, 0, slot, v); }
+  __device__ __forceinline__ static F3Acc get(float (*lds)[256], u32 slot) { return F3Acc{getF3(lds, 0, slot)}; }
+};
 
 // neighbour record of the velocity sweeps: position/tag and velocity/volume
 struct PV {
@@ -211,6 +320,59 @@ __device__ __forceinline__ void momentsZero(Moments& m) {
   m.s = splat3(0.0f);
   m.q00 = m.q11 = m.q22 = m.q01 = m.q02 = m.q12 = 0.0f;
 }
+// the moments as a runPasses accumulator (kWithQ = false: M only)
+template <bool kWithQ>
+struct MomAcc {
+  Moments m;
+  __device__ __forceinline__ static MomAcc zero() {
+    MomAcc a;
+    momentsZero(a.m);
+    return a;
+  }
+  __device__ __forceinline__ void quadSum() {
+#pragma unroll
+    for (int c = 0; c < 3; c++) m.M.c[c] = xpbi::quadSum(m.M.c[c]);
+    if (kWithQ) {
+      m.s = xpbi::quadSum(m.s);
+      m.q00 = xpbi::quadSum(m.q00);
+      m.q11 = xpbi::quadSum(m.q11);
+      m.q22 = xpbi::quadSum(m.q22);
+      m.q01 = xpbi::quadSum(m.q01);
+      m.q02 = xpbi::quadSum(m.q02);
+      m.q12 = xpbi::quadSum(m.q12);
+    }
+  }
+  static constexpr int kF = kWithQ ? 18 : 9;
+  __device__ __forceinline__ void put(float (*lds)[256], u32 slot) const {
+#pragma unroll
+    for (int c = 0; c < 3; c++) putF3(lds, 3 * c, slot, m.M.c[c]);
+    if (kWithQ) {
+      putF3(lds, 9, slot, m.s);
+      lds[12][slot] = m.q00;
+      lds[13][slot] = m.q11;
+      lds[14][slot] = m.q22;
+      lds[15][slot] = m.q01;
+      lds[16][slot] = m.q02;
+      lds[17][slot] = m.q12;
+    }
+  }
+  __device__ __forceinline__ static MomAcc get(float (*lds)[256], u32 slot) {
+    MomAcc a;
+    momentsZero(a.m);
+#pragma unroll
+    for (int c = 0; c < 3; c++) a.m.M.c[c] = getF3(lds, 3 * c, slot);
+    if (kWithQ) {
+      a.m.s = getF3(lds, 9, slot);
+      a.m.q00 = lds[12][slot];
+      a.m.q11 = lds[13][slot];
+      a.m.q22 = lds[14][slot];
+      a.m.q01 = lds[15][slot];
+      a.m.q02 = lds[16][slot];
+      a.m.q12 = lds[17][slot];
+    }
+    return a;
+  }
+};
 // one neighbour's contribution: g = gradW(x_i - x_b), w = V_b (v_b - v_i)
 template <bool kWithQ>
 __device__ __forceinline__ void momentsAdd(Moments& m, f3 g, f3 w, float Vb, float invMb) {
@@ -227,15 +389,14 @@ __device__ __forceinline__ void momentsAdd(Moments& m, f3 g, f3 w, float Vb, flo
     m.q12 += cg.y * g.z;
   }
 }
-// kAllSolid: the scene has no water, every neighbour contributes
+// lane q's share of particle i's moments (kAllSolid: the scene has no water,
+// every neighbour contributes); the quad's shares sum to them
 template <bool kWithQ, bool kAllSolid>
-__device__ __forceinline__ Moments neighbourMoments(const KP& P, const u32* __restrict__ cs,
-                                                    const float4* __restrict__ pos, const float4* __restrict__ vin,
-                                                    const NLc& nl, const float* invMass, u32 solidMask, u32 i,
-                                                    f3 xi, f3 vi) {
-  Moments m;
-  momentsZero(m);
-  forSupportG(P, cs, nl, i, xi, [&](u32 b) { return PV{gat(pos, b), gat(vin, b)}; },
+__device__ __forceinline__ void neighbourMomentsQ(const KP& P, const u32* __restrict__ cs,
+                                                  const float4* __restrict__ pos, const float4* __restrict__ vin,
+                                                  const NLc& nl, const float* invMass, u32 solidMask, u32 i, u32 q,
+                                                  f3 xi, f3 vi, Moments& m) {
+  forSupport(P, cs, pos, nl, i, xi, q, [&](u32 b) { return PV{gat(pos, b), gat(vin, b)}; },
               [&](u32 b, const PV& r, f3 dxv, float r2) {
                 const float4 pb = r.p, vb = r.v;
                 if (kAllSolid || solidTag(solidMask, pb.w)) {
@@ -243,7 +404,25 @@ __device__ __forceinline__ Moments neighbourMoments(const KP& P, const u32* __re
                                      invMass[matOfTag(pb.w)]);
                 }
               });
-  return m;
+}
+// the moments of the particles of a block's run, through runPasses: each thread
+// returns those of particle first + threadIdx.x (zero for water)
+template <bool kWithQ>
+__device__ __forceinline__ Moments runMoments(const KP& P, const MatLds& T, u32 first, const u32* __restrict__ cs,
+                                              const float4* __restrict__ pos, const float4* __restrict__ vin,
+                                              const NLc& nl) {
+  __shared__ float lds[MomAcc<kWithQ>::kF][256];
+  return runPasses<MomAcc<kWithQ>>(lds, first, P.pEnd, [&](u32 i, u32 q, MomAcc<kWithQ>& a) {
+           const float4 pi = pos[i];
+           if (!((T.solidMask >> matOfTag(pi.w)) & 1u)) return;
+           const f3 vi = xyz(vin[i]);
+           if (allSolid(T))
+             neighbourMomentsQ<kWithQ, true>(P, cs, pos, vin, nl, T.invMass, T.solidMask, i, q, xyz(pi), vi,
+                                                     a.m);
+           else
+             neighbourMomentsQ<kWithQ, false>(P, cs, pos, vin, nl, T.invMass, T.solidMask, i, q, xyz(pi), vi,
+                                                      a.m);
+         }).m;
 }
 
 // Frobenius product A : B of two column-major matrices
@@ -399,7 +578,7 @@ __device__ __forceinline__ void solveBPair(const KP& P, const MatLds& T, float V
     *dv = *dv + Vi * mul(Spb, gradWfast(P, dxp, r2));
     const f3 xcb = xyz(pb4) + P.dt * xyz(vb);
     const f3 dxc = xci - xcb;
-    const float d2 = dot(dxc, dxc);
+    const float d2 = dist2(dxc);
     const float minD = P.minD;
     if (d2 < minD * minD && d2 > 1e-18f) {
       float idist = xrsq(d2);
@@ -410,24 +589,38 @@ __device__ __forceinline__ void solveBPair(const KP& P, const MatLds& T, float V
   }
 }
 
+// lane q's share of the constraint gather dv += V_i Sp_b gradW(x_b - x_i) and
+// of the contact correction (selfDv is added by the owner of the particle)
+struct SolveBAcc {
+  static constexpr int kF = 6;
+  f3 dv, corr;
+  __device__ __forceinline__ static SolveBAcc zero() { return SolveBAcc{splat3(0.0f), splat3(0.0f)}; }
+  __device__ __forceinline__ void quadSum() {
+    dv = xpbi::quadSum(dv);
+    corr = xpbi::quadSum(corr);
+  }
+  __device__ __forceinline__ void put(float (*lds)[256], u32 slot) const {
+    putF3(lds, 0, slot, dv);
+    putF3(lds, 3, slot, corr);
+  }
+  __device__ __forceinline__ static SolveBAcc get(float (*lds)[256], u32 slot) {
+    return SolveBAcc{getF3(lds, 0, slot), getF3(lds, 3, slot)};
+  }
+};
 template <bool kAllSolid>
-__device__ __forceinline__ void solveBGatherGlobal(const KP& P, const MatLds& T, u32 i, float4 pi, float4 vi4,
-                                                   const float4* __restrict__ pos, const float4* __restrict__ vin,
-                                                   M3C Sp, const float4* __restrict__ selfDv,
-                                                   const u32* __restrict__ cs, const NLc& nl, f3* dv, f3* corr) {
-  *dv = xyz(selfDv[i]);
+__device__ __forceinline__ void solveBGatherQ(const KP& P, const MatLds& T, u32 i, u32 q, float4 pi, float4 vi4,
+                                              const float4* __restrict__ pos, const float4* __restrict__ vin, M3C Sp,
+                                              const u32* __restrict__ cs, const NLc& nl, SolveBAcc& a) {
   const float Vi = vi4.w;
   const f3 xi = xyz(pi);
   const f3 xci = xi + P.dt * xyz(vi4);
   const float* wc_i = T.wc[matOfTag(pi.w)];
-  // indexed by pointer, not by gat(): the compiler then issues a batch's loads
-  // grouped by array, which measured 30% faster for these five arrays than the
-  // per-neighbour interleaving it emits for 32-bit offsets
-  forSupportG<kBatchB>(P, cs, nl, i, xi, [&](u32 b) { return SpGather{pos[b], vin[b], Sp.a[b], Sp.b[b], Sp.c[b]}; },
-              [&](u32 b, const SpGather& g, f3 dxv, float r2) {
-                const f33 Spb{{f3{g.a.x, g.a.y, g.a.z}, f3{g.a.w, g.b.x, g.b.y}, f3{g.b.z, g.b.w, g.c}}};
-                solveBPair<kAllSolid>(P, T, Vi, wc_i, xci, dxv, r2, g.p, g.v, Spb, dv, corr);
-              });
+  forSupport(P, cs, pos, nl, i, xi, q,
+      [&](u32 b) { return SpGather{pos[b], vin[b], Sp.a[b], Sp.b[b], Sp.c[b]}; },
+      [&](u32 b, const SpGather& g, f3 dxv, float r2) {
+        const f33 Spb{{f3{g.a.x, g.a.y, g.a.z}, f3{g.a.w, g.b.x, g.b.y}, f3{g.b.z, g.b.w, g.c}}};
+        solveBPair<kAllSolid>(P, T, Vi, wc_i, xci, dxv, r2, g.p, g.v, Spb, &a.dv, &a.corr);
+      });
 }
 
 // simkernels.js:357-365: impulse cap, contact relaxation, boundaries; with xcOut

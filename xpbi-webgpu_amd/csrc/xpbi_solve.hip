@@ -69,7 +69,7 @@ __global__ __launch_bounds__(256) void k_computeL(const KP* __restrict__ kps, Wo
 #pragma unroll
           for (int j = 0; j < 8; j++) {
             d[j] = xi - xyz(pb[j]);
-            r2[j] = dot(d[j], d[j]);
+            r2[j] = dist2(d[j]);
             ok[j] = (b0 + j < end) && (b0 + j != i) && (r2[j] < P.supp2);
           }
           float Vb[8];
@@ -85,7 +85,7 @@ __global__ __launch_bounds__(256) void k_computeL(const KP* __restrict__ kps, Wo
               flag |= off > kNbrOffMax ? kNbrFlag : 0u;
               const u16 e = (u16)off;
               if (n < kStage) stage[n][tid] = e;
-              else if (n < K) nLst[(size_t)n * NT + i] = e;
+              else if (n < K) nLst[slotAt(n, i, NT)] = e;
               n++;
             }
           }
@@ -94,8 +94,21 @@ __global__ __launch_bounds__(256) void k_computeL(const KP* __restrict__ kps, Wo
     }
     // the particle's own offset must fit too (the sweeps' batch tails gather i)
     flag |= (i - base) > kNbrOffMax ? kNbrFlag : 0u;
+    // the staged slots, one group of 16 (32 bytes, in the list's interleaved
+    // order: position p holds slot 16 g + 4 (p mod 4) + p / 4) at a time
     const u32 ns = min(n, min(kStage, K));  // the list may hold fewer slots than the stage
-    for (u32 k = 0; k < ns; k++) nLst[(size_t)k * NT + i] = stage[k][tid];
+    for (u32 g = 0; 16u * g < ns; g++) {
+      u32 w[8];
+#pragma unroll
+      for (int p = 0; p < 8; p++) {
+        const u32 p0 = 2u * p, p1 = 2u * p + 1u;
+        w[p] = (u32)stage[16u * g + 4u * (p0 & 3u) + (p0 >> 2)][tid] |
+               ((u32)stage[16u * g + 4u * (p1 & 3u) + (p1 >> 2)][tid] << 16);
+      }
+      uint4* dst = (uint4*)(nLst + ((size_t)g * NT + i) * 16u);
+      dst[0] = make_uint4(w[0], w[1], w[2], w[3]);
+      dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
+    }
     nHead[i] = make_uint2(base, n | flag);
     SVD3<double> s = svd3<double>(mcast<double>(A));
     d3 inv = splat3(0.0);
@@ -123,16 +136,13 @@ __global__ __launch_bounds__(256) void k_solveA(const KP* __restrict__ kps, Work
   forWork<kPersist>(W, [&](u32 scene, u32 first) {
     sceneMats(T, mats, scene, &curScene);
     const KP& P = kps[scene];
+    const Moments mo = runMoments<true>(P, T, first, cs, pos, vin, nl);
     const u32 i = first + threadIdx.x;
     if (i >= P.pEnd) return;
     const float4 pi = pos[i];
     const f3 vi = xyz(vin[i]);
-    Moments mo;
     const u32 mi = matOfTag(pi.w);
     const bool solid = (T.solidMask >> mi) & 1u;
-    if (solid)
-      mo = allSolid(T) ? neighbourMoments<true, true>(P, cs, pos, vin, nl, T.invMass, T.solidMask, i, xyz(pi), vi)
-                       : neighbourMoments<true, false>(P, cs, pos, vin, nl, T.invMass, T.solidMask, i, xyz(pi), vi);
     const float hint = (hbHint && T.m[mi].kind == MAT_HB) ? hbHint[i] : -1.0f;
     solveAFinish(P, T, i, pi, vi, solid, mo, Fm, Lm, aux, Sp, selfDv, &r2, &c2, hint);
   });
@@ -220,17 +230,21 @@ __global__ __launch_bounds__(256) void k_solveB(const KP* __restrict__ kps, Work
   forWork<kPersist>(W, [&](u32 scene, u32 first) {
     sceneMats(T, mats, scene, &curScene);
     const KP& P = kps[scene];
+    __shared__ float lds[SolveBAcc::kF][256];
+    const SolveBAcc a = runPasses<SolveBAcc>(lds, first, P.pEnd, [&](u32 i, u32 q, SolveBAcc& acc) {
+      const float4 pi = pos[i];
+      if (!((T.solidMask >> matOfTag(pi.w)) & 1u)) return;
+      const float4 vi4 = vin[i];
+      if (allSolid(T)) solveBGatherQ<true>(P, T, i, q, pi, vi4, pos, vin, Sp, cs, nl, acc);
+      else solveBGatherQ<false>(P, T, i, q, pi, vi4, pos, vin, Sp, cs, nl, acc);
+    });
     const u32 i = first + threadIdx.x;
     if (i >= P.pEnd) return;
     const float4 pi = pos[i];
     const float4 vi4 = vin[i];
-    const u32 mi = matOfTag(pi.w);
-    f3 dv = splat3(0.0f), corr = splat3(0.0f);
-    if ((T.solidMask >> mi) & 1u) {
-      if (allSolid(T)) solveBGatherGlobal<true>(P, T, i, pi, vi4, pos, vin, Sp, selfDv, cs, nl, &dv, &corr);
-      else solveBGatherGlobal<false>(P, T, i, pi, vi4, pos, vin, Sp, selfDv, cs, nl, &dv, &corr);
-    }
-    solveBFinish(P, T, i, pi, vi4, dv, corr, vout, xcOut);
+    f3 dv = splat3(0.0f);
+    if ((T.solidMask >> matOfTag(pi.w)) & 1u) dv = xyz(selfDv[i]) + a.dv;
+    solveBFinish(P, T, i, pi, vi4, dv, a.corr, vout, xcOut);
   });
 }
 
@@ -341,28 +355,30 @@ __global__ __launch_bounds__(256) void k_xsph(const KP* __restrict__ kps, WorkTa
   forWork<kPersist>(W, [&](u32 scene, u32 first) {
     sceneMats(T, mats, scene, &curScene);
     const KP& P = kps[scene];
+    __shared__ float lds[F3Acc::kF][256];
+    const F3Acc a = runPasses<F3Acc>(lds, first, P.pEnd, [&](u32 i, u32 q, F3Acc& acc) {
+      const float4 pi = pos[i];
+      const f3 vi = xyz(vin[i]);
+      const bool iSolid = solidTag(T.solidMask, pi.w);
+      // kAll: no water in the scene, every neighbour shares the particle's phase
+      auto gather = [&](auto all) {
+        constexpr bool kAll = decltype(all)::value;
+        forSupport(P, cs, pos, nl, i, xyz(pi), q, [&](u32 b) { return PV{gat(pos, b), gat(vin, b)}; },
+                    [&](u32 b, const PV& r, f3 dxv, float r2) {
+                      if (kAll || solidTag(T.solidMask, r.p.w) == iSolid)
+                        acc.v = acc.v + r.v.w * (xyz(r.v) - vi) * kernelWfast(P, r2);
+                    });
+      };
+      if (allSolid(T)) gather(std::true_type{});
+      else gather(std::false_type{});
+    });
     const u32 i = first + threadIdx.x;
     if (i >= P.pEnd) return;
     const float4 pi = pos[i];
-    const f3 xi = xyz(pi);
     const float4 vi4 = vin[i];
-    const f3 vi = xyz(vi4);
-    const bool iSolid = solidTag(T.solidMask, pi.w);
-    f3 acc = splat3(0.0f);
-    // kAll: no water in the scene, every neighbour shares the particle's phase
-    auto gather = [&](auto all) {
-      constexpr bool kAll = decltype(all)::value;
-      forSupportG(P, cs, nl, i, xi, [&](u32 b) { return PV{gat(pos, b), gat(vin, b)}; },
-                  [&](u32 b, const PV& r, f3 dxv, float r2) {
-                    if (kAll || solidTag(T.solidMask, r.p.w) == iSolid)
-                      acc = acc + r.v.w * (xyz(r.v) - vi) * kernelWfast(P, r2);
-                  });
-    };
-    if (allSolid(T)) gather(std::true_type{});
-    else gather(std::false_type{});
     float c = P.xsphC;
-    if (!iSolid) c = P.xsphC * 10.0f;
-    f3 v = vi + c * acc;
+    if (!solidTag(T.solidMask, pi.w)) c = P.xsphC * 10.0f;
+    const f3 v = xyz(vi4) + c * a.v;
     vout[i] = make_float4(v.x, v.y, v.z, vi4.w);
   });
 }
@@ -379,6 +395,7 @@ __global__ __launch_bounds__(256) void k_updateF(const KP* __restrict__ kps, Wor
   forWork<kPersist>(W, [&](u32 scene, u32 first) {
     sceneMats(T, mats, scene, &curScene);
     const KP& P = kps[scene];
+    const Moments mo = runMoments<false>(P, T, first, cs, pos, vin, nl);
     const u32 i = first + threadIdx.x;
     if (i >= P.pEnd) return;
     const float4 pi = pos[i];
@@ -386,11 +403,7 @@ __global__ __launch_bounds__(256) void k_updateF(const KP* __restrict__ kps, Wor
     if (!((T.solidMask >> mi) & 1u)) return;
     const float state = aux[i].z;
     MatV m = hardenedMat(matv(T, mi), state);
-    const f3 xi = xyz(pi);
     const f33 Li = ldm(Lm, i);
-    const f3 vi = xyz(vin[i]);
-    const Moments mo = allSolid(T) ? neighbourMoments<false, true>(P, cs, pos, vin, nl, T.invMass, T.solidMask, i, xi, vi)
-                                   : neighbourMoments<false, false>(P, cs, pos, vin, nl, T.invMass, T.solidMask, i, xi, vi);
     const f33 gradV = mul(mo.M, transpose(Li));
     f33 Ftr = mul(madd(mident<float>(), mscale(gradV, P.dt)), ldm(Fm, i));
     SVD3<float> sv = svd3_guarded(Ftr);
@@ -449,6 +462,7 @@ __global__ __launch_bounds__(256) void k_features(const KP* __restrict__ kps, Wo
   forWork<kPersist>(W, [&](u32 scene, u32 first) {
     sceneMats(T, mats, scene, &curScene);
     const KP& P = kps[scene];
+    const Moments mo = runMoments<true>(P, T, first, cs, pos, vin, nl);
     const u32 i = first + threadIdx.x;
     if (i >= P.pEnd) return;
     float row[16];
@@ -459,10 +473,6 @@ __global__ __launch_bounds__(256) void k_features(const KP* __restrict__ kps, Wo
     const u32 mi = matOfTag(pi.w);
     bool wrote = false;
     if ((T.solidMask >> mi) & 1u) {
-      const f3 vi = xyz(vin[i]);
-      const Moments mo =
-          allSolid(T) ? neighbourMoments<true, true>(P, cs, pos, vin, nl, T.invMass, T.solidMask, i, xyz(pi), vi)
-                      : neighbourMoments<true, false>(P, cs, pos, vin, nl, T.invMass, T.solidMask, i, xyz(pi), vi);
       const float4 a = aux[i];
       const Constraint c = evalConstraint(P, T, i, pi, a, mo, Fm, Lm, -1.0f, true);
       if (c.active) {

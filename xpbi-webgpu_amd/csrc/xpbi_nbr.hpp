@@ -101,8 +101,9 @@ __device__ __forceinline__ float dist2(f3 d) { return fmaf(d.z, d.z, fmaf(d.y, d
 
 // A walk over the particle's cell rows that yields its support neighbours one at
 // a time, in the reference's candidate order: the fallback of a particle whose
-// list overflowed.  next() returns the next neighbour of lane q (neighbour
-// number c belongs to lane c mod 4), or ~0u once the rows are exhausted.
+// list overflowed.  next() returns the next neighbour of lane q of kL (neighbour
+// number c belongs to lane c mod kL), or ~0u once the rows are exhausted.
+template <u32 kL>
 struct RowCursor {
   i3 cc;
   u32 x0, x1;
@@ -134,7 +135,7 @@ struct RowCursor {
       const u32 c = b++;
       if (c == i) continue;
       if (dist2(xi - xyz(pos[c])) < P.supp2) {
-        const bool mine = (cnt & 3u) == q;
+        const bool mine = (cnt & (kL - 1u)) == q;
         cnt++;
         if (mine) return c;
       }
@@ -142,96 +143,108 @@ struct RowCursor {
   }
 };
 
-// Support-neighbour walk of particle i by the four lanes of a quad: lane q
-// (0..3) takes the neighbours numbered q, q + 4, q + 8, ..., so that one
-// wave-wide gather reads four consecutive neighbours (mostly adjacent records of
-// one cell row) of each of 16 consecutive particles, and each lane reads its
-// list entries as one 8-byte word per four neighbours.  load(b) gathers a
-// neighbour's record, whose member p is (x_b, tag_b); use(b, rec, dxv = x_i -
-// x_b, |dxv|^2) consumes it, in the lane's order; the four records of a word are
-// loaded together.  A particle whose list overflowed takes its neighbours from a
-// RowCursor instead: the same neighbours in the same lanes and order, through
-// the same single copy of the loads and of use().
-template <class Load, class Use>
+// Support-neighbour walk of particle i by kL = 4 or 8 lanes: lane q (0..kL-1)
+// takes the neighbours numbered q, q + kL, q + 2 kL, ..., so that one wave-wide
+// gather reads kL consecutive neighbours (mostly adjacent records of one cell
+// row) of each of 64 / kL consecutive particles.  A lane reads the list as one
+// 8-byte word per group of 16 slots: with kL = 4 the word of lane q holds its
+// slots 16 g + q + 4 j (j = 0..3); with kL = 8 lane q reads the word of lane
+// q mod 4 and takes its entries q / 4 and q / 4 + 2 (slots 16 g + q and
+// 16 g + q + 8).  load(b) gathers a neighbour's record, whose member p is (x_b,
+// tag_b); use(b, rec, dxv = x_i - x_b, |dxv|^2) consumes it, in the lane's
+// order; the records of a word are loaded together.  A particle whose list
+// overflowed takes its neighbours from a RowCursor instead: the same neighbours
+// in the same lanes and order, through the same single copy of the loads and
+// of use().
+template <u32 kL, class Load, class Use>
 __device__ __forceinline__ void forSupport(const KP& P, const u32* __restrict__ cs, const float4* __restrict__ pos,
                                            const NLc& nl, u32 i, f3 xi, u32 q, Load&& load, Use&& use) {
+  static_assert(kL == 4u || kL == 8u, "four or eight lanes per particle");
+  constexpr int kE = 16 / (int)kL;  // the lane's slots per group
   const uint2 hd = nl.head[i];
   const u32 n = hd.y;
   const bool rows = n > nl.K;
-  // list: the lane's word of group g (slots 16 g + 4 j + q, j = 0..3) is 8-byte
-  // word (g N + i) 4 + q; the next group's word is prefetched
-  const uint64_t* __restrict__ lw = (const uint64_t*)nl.lst + ((size_t)i * 4u + q);
+  // list: the lane's word of group g is 8-byte word (g N + i) 4 + q mod 4; the
+  // next group's word is prefetched
+  const uint64_t* __restrict__ lw = (const uint64_t*)nl.lst + ((size_t)i * 4u + (q & 3u));
   const size_t gstep = (size_t)nl.N * 4u;  // 8-byte words per group
-  u32 s0 = q;                              // the lane's first slot of the group
+  const u32 sh = kL == 8u ? 16u * (q >> 2) : 0u;  // the lane's first entry within the word
+  u32 s0 = q;                                     // the lane's first slot of the group
   uint64_t w = (!rows && s0 < n) ? lw[0] : 0ull;
-  RowCursor rc;
+  RowCursor<kL> rc;
   if (rows) rc.init(P, xi);
 #pragma unroll 1
   for (u32 g = 0;; g++) {
-    u32 b[4];
-    bool v[4];
+    u32 b[kE];
+    bool v[kE];
     if (!rows) {
       if (s0 >= n) break;
       const uint64_t wn = (s0 + 16u < n) ? lw[(size_t)(g + 1u) * gstep] : 0ull;
+      const uint64_t ws = w >> sh;
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        v[j] = s0 + 4u * (u32)j < n;
-        b[j] = v[j] ? hd.x + (u32)((w >> (16 * j)) & 0xffffu) : i;
+      for (int j = 0; j < kE; j++) {
+        v[j] = s0 + kL * (u32)j < n;
+        b[j] = v[j] ? hd.x + (u32)((ws >> (16 * (16 / kE / 4) * j)) & 0xffffu) : i;
       }
       w = wn;
       s0 += 16u;
     } else {
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
+      for (int j = 0; j < kE; j++) {
         const u32 c = rc.next(P, cs, pos, i, xi, q);
         v[j] = c != ~0u;
         b[j] = v[j] ? c : i;
       }
       if (!v[0]) break;
     }
-    decltype(load(0u)) d[4];
+    decltype(load(0u)) d[kE];
 #pragma unroll
-    for (int j = 0; j < 4; j++) d[j] = load(b[j]);
+    for (int j = 0; j < kE; j++) d[j] = load(b[j]);
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
+    for (int j = 0; j < kE; j++) {
       if (v[j]) {
         const f3 dd = xi - xyz(d[j].p);
         use(b[j], d[j], dd, dist2(dd));
       }
     }
-    if (!v[3]) break;
+    if (!v[kE - 1]) break;
   }
 }
 
-// sum over the four lanes of a quad, ((l0 + l1) + (l2 + l3)) in every lane (the
-// two xor steps add the same pairs in every lane: f32 addition commutes)
-__device__ __forceinline__ float quadSum(float x) {
+// sum over the kL lanes (4 or 8) of a particle: ((l0 + l1) + (l2 + l3)) for a
+// quad, that plus the same of lanes 4..7 for eight; every lane gets the same
+// value (each step adds the same pair in every lane: f32 addition commutes)
+template <u32 kL>
+__device__ __forceinline__ float laneSum(float x) {
   x += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xB1, 0xF, 0xF, false));  // quad_perm 1,0,3,2
   x += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x4E, 0xF, 0xF, false));  // quad_perm 2,3,0,1
+  if (kL == 8u) x += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x141, 0xF, 0xF, false));  // row_half_mirror
   return x;
 }
-__device__ __forceinline__ f3 quadSum(f3 v) { return f3{quadSum(v.x), quadSum(v.y), quadSum(v.z)}; }
+template <u32 kL>
+__device__ __forceinline__ f3 laneSum(f3 v) { return f3{laneSum<kL>(v.x), laneSum<kL>(v.y), laneSum<kL>(v.z)}; }
 
-// The per-particle sums of a block's run of up to 256 particles, in four passes
-// of 64: in pass p the quad of lanes 4 g .. 4 g + 3 gathers particle
-// first + 64 p + g (gather(i, q, acc) adds lane q's share), the quad's shares
-// are summed, and the quad's first lane parks the sum in LDS (lds: Acc::kF rows
-// of 256 floats, one column per particle of the run).  Returns the sum of
-// particle first + threadIdx.x, which the thread then finishes.  Acc provides
-// zero(), quadSum(), put(lds, slot) and get(lds, slot).  Contains block
-// barriers: every thread of the block calls it.
-template <class Acc, class Gather>
+// The per-particle sums of a block's run of up to 256 particles, gathered by kL
+// lanes per particle in kL passes of 256 / kL: in pass p the lanes kL g ..
+// kL g + kL - 1 gather particle first + (256 / kL) p + g (gather(i, q, acc) adds
+// lane q's share), the shares are summed, and the group's first lane parks the
+// sum in LDS (lds: Acc::kF rows of 256 floats, one column per particle of the
+// run).  Returns the sum of particle first + threadIdx.x, which the thread then
+// finishes.  Acc provides zero(), laneSum<kL>(), put(lds, slot) and
+// get(lds, slot).  Contains block barriers: every thread of the block calls it.
+template <u32 kL, class Acc, class Gather>
 __device__ __forceinline__ Acc runPasses(float (*lds)[256], u32 first, u32 pEnd, Gather&& gather) {
-  const u32 q = threadIdx.x & 3u, g = threadIdx.x >> 2;
+  constexpr u32 kPer = 256u / kL;  // particles per pass
+  const u32 q = threadIdx.x & (kL - 1u), g = threadIdx.x / kL;
 #pragma unroll 1
-  for (u32 pass = 0; pass < 4u; pass++) {
-    const u32 base = first + 64u * pass;
+  for (u32 pass = 0; pass < kL; pass++) {
+    const u32 base = first + kPer * pass;
     if (base >= pEnd) break;
     const u32 i = base + g;
     Acc a = Acc::zero();
     if (i < pEnd) gather(i, q, a);
-    a.quadSum();
-    if (q == 0u) a.put(lds, 64u * pass + g);
+    a.template laneSum<kL>();
+    if (q == 0u) a.put(lds, kPer * pass + g);
   }
   __syncthreads();
   const Acc own = Acc::get(lds, threadIdx.x);
@@ -251,7 +264,7 @@ struct F3Acc {
   static constexpr int kF = 3;
   f3 v;
   __device__ __forceinline__ static F3Acc zero() { return F3Acc{splat3(0.0f)}; }
-  __device__ __forceinline__ void quadSum() { v = xpbi::quadSum(v); }
+  template <u32 kL> __device__ __forceinline__ void laneSum() { v = xpbi::laneSum<kL>(v); }
   __device__ __forceinline__ void put(float (*lds)[256], u32 slot) const { putF3(lds, 0, slot, v); }
   __device__ __forceinline__ static F3Acc get(float (*lds)[256], u32 slot) { return F3Acc{getF3(lds, 0, slot)}; }
 };
@@ -329,17 +342,17 @@ struct MomAcc {
     momentsZero(a.m);
     return a;
   }
-  __device__ __forceinline__ void quadSum() {
+  template <u32 kL> __device__ __forceinline__ void laneSum() {
 #pragma unroll
-    for (int c = 0; c < 3; c++) m.M.c[c] = xpbi::quadSum(m.M.c[c]);
+    for (int c = 0; c < 3; c++) m.M.c[c] = xpbi::laneSum<kL>(m.M.c[c]);
     if (kWithQ) {
-      m.s = xpbi::quadSum(m.s);
-      m.q00 = xpbi::quadSum(m.q00);
-      m.q11 = xpbi::quadSum(m.q11);
-      m.q22 = xpbi::quadSum(m.q22);
-      m.q01 = xpbi::quadSum(m.q01);
-      m.q02 = xpbi::quadSum(m.q02);
-      m.q12 = xpbi::quadSum(m.q12);
+      m.s = xpbi::laneSum<kL>(m.s);
+      m.q00 = xpbi::laneSum<kL>(m.q00);
+      m.q11 = xpbi::laneSum<kL>(m.q11);
+      m.q22 = xpbi::laneSum<kL>(m.q22);
+      m.q01 = xpbi::laneSum<kL>(m.q01);
+      m.q02 = xpbi::laneSum<kL>(m.q02);
+      m.q12 = xpbi::laneSum<kL>(m.q12);
     }
   }
   static constexpr int kF = kWithQ ? 18 : 9;
@@ -391,12 +404,12 @@ __device__ __forceinline__ void momentsAdd(Moments& m, f3 g, f3 w, float Vb, flo
 }
 // lane q's share of particle i's moments (kAllSolid: the scene has no water,
 // every neighbour contributes); the quad's shares sum to them
-template <bool kWithQ, bool kAllSolid>
+template <u32 kL, bool kWithQ, bool kAllSolid>
 __device__ __forceinline__ void neighbourMomentsQ(const KP& P, const u32* __restrict__ cs,
                                                   const float4* __restrict__ pos, const float4* __restrict__ vin,
                                                   const NLc& nl, const float* invMass, u32 solidMask, u32 i, u32 q,
                                                   f3 xi, f3 vi, Moments& m) {
-  forSupport(P, cs, pos, nl, i, xi, q, [&](u32 b) { return PV{gat(pos, b), gat(vin, b)}; },
+  forSupport<kL>(P, cs, pos, nl, i, xi, q, [&](u32 b) { return PV{gat(pos, b), gat(vin, b)}; },
               [&](u32 b, const PV& r, f3 dxv, float r2) {
                 const float4 pb = r.p, vb = r.v;
                 if (kAllSolid || solidTag(solidMask, pb.w)) {
@@ -407,20 +420,20 @@ __device__ __forceinline__ void neighbourMomentsQ(const KP& P, const u32* __rest
 }
 // the moments of the particles of a block's run, through runPasses: each thread
 // returns those of particle first + threadIdx.x (zero for water)
-template <bool kWithQ>
+template <u32 kL, bool kWithQ>
 __device__ __forceinline__ Moments runMoments(const KP& P, const MatLds& T, u32 first, const u32* __restrict__ cs,
                                               const float4* __restrict__ pos, const float4* __restrict__ vin,
                                               const NLc& nl) {
   __shared__ float lds[MomAcc<kWithQ>::kF][256];
-  return runPasses<MomAcc<kWithQ>>(lds, first, P.pEnd, [&](u32 i, u32 q, MomAcc<kWithQ>& a) {
+  return runPasses<kL, MomAcc<kWithQ>>(lds, first, P.pEnd, [&](u32 i, u32 q, MomAcc<kWithQ>& a) {
            const float4 pi = pos[i];
            if (!((T.solidMask >> matOfTag(pi.w)) & 1u)) return;
            const f3 vi = xyz(vin[i]);
            if (allSolid(T))
-             neighbourMomentsQ<kWithQ, true>(P, cs, pos, vin, nl, T.invMass, T.solidMask, i, q, xyz(pi), vi,
+             neighbourMomentsQ<kL, kWithQ, true>(P, cs, pos, vin, nl, T.invMass, T.solidMask, i, q, xyz(pi), vi,
                                                      a.m);
            else
-             neighbourMomentsQ<kWithQ, false>(P, cs, pos, vin, nl, T.invMass, T.solidMask, i, q, xyz(pi), vi,
+             neighbourMomentsQ<kL, kWithQ, false>(P, cs, pos, vin, nl, T.invMass, T.solidMask, i, q, xyz(pi), vi,
                                                       a.m);
          }).m;
 }
@@ -595,9 +608,9 @@ struct SolveBAcc {
   static constexpr int kF = 6;
   f3 dv, corr;
   __device__ __forceinline__ static SolveBAcc zero() { return SolveBAcc{splat3(0.0f), splat3(0.0f)}; }
-  __device__ __forceinline__ void quadSum() {
-    dv = xpbi::quadSum(dv);
-    corr = xpbi::quadSum(corr);
+  template <u32 kL> __device__ __forceinline__ void laneSum() {
+    dv = xpbi::laneSum<kL>(dv);
+    corr = xpbi::laneSum<kL>(corr);
   }
   __device__ __forceinline__ void put(float (*lds)[256], u32 slot) const {
     putF3(lds, 0, slot, dv);
@@ -607,7 +620,7 @@ struct SolveBAcc {
     return SolveBAcc{getF3(lds, 0, slot), getF3(lds, 3, slot)};
   }
 };
-template <bool kAllSolid>
+template <u32 kL, bool kAllSolid>
 __device__ __forceinline__ void solveBGatherQ(const KP& P, const MatLds& T, u32 i, u32 q, float4 pi, float4 vi4,
                                               const float4* __restrict__ pos, const float4* __restrict__ vin, M3C Sp,
                                               const u32* __restrict__ cs, const NLc& nl, SolveBAcc& a) {
@@ -615,7 +628,7 @@ __device__ __forceinline__ void solveBGatherQ(const KP& P, const MatLds& T, u32 
   const f3 xi = xyz(pi);
   const f3 xci = xi + P.dt * xyz(vi4);
   const float* wc_i = T.wc[matOfTag(pi.w)];
-  forSupport(P, cs, pos, nl, i, xi, q,
+  forSupport<kL>(P, cs, pos, nl, i, xi, q,
       [&](u32 b) { return SpGather{pos[b], vin[b], Sp.a[b], Sp.b[b], Sp.c[b]}; },
       [&](u32 b, const SpGather& g, f3 dxv, float r2) {
         const f33 Spb{{f3{g.a.x, g.a.y, g.a.z}, f3{g.a.w, g.b.x, g.b.y}, f3{g.b.z, g.b.w, g.c}}};

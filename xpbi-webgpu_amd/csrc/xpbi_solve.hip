@@ -18,6 +18,24 @@
 
 namespace xpbi {
 
+// lanes per particle of each sweep's neighbour walk (forSupport: 4 or 8)
+#ifndef XPBI_LANES_A
+#define XPBI_LANES_A 4
+#endif
+#ifndef XPBI_LANES_B
+#define XPBI_LANES_B 4
+#endif
+#ifndef XPBI_LANES_X
+#define XPBI_LANES_X 4
+#endif
+#ifndef XPBI_LANES_F
+#define XPBI_LANES_F 4
+#endif
+constexpr u32 kLanesA = XPBI_LANES_A;  // solveA and the warm-start features pass
+constexpr u32 kLanesB = XPBI_LANES_B;  // solveB
+constexpr u32 kLanesX = XPBI_LANES_X;  // xsph
+constexpr u32 kLanesF = XPBI_LANES_F;  // updateF
+
 // ---------------------------------------------------------------- computeL
 // simkernels.js:189-221.  A is accumulated in f32; the SVD pseudo-inverse with
 // absolute tolerance 0.2 runs in FP64.  The same candidate walk records the
@@ -136,7 +154,7 @@ __global__ __launch_bounds__(256) void k_solveA(const KP* __restrict__ kps, Work
   forWork<kPersist>(W, [&](u32 scene, u32 first) {
     sceneMats(T, mats, scene, &curScene);
     const KP& P = kps[scene];
-    const Moments mo = runMoments<true>(P, T, first, cs, pos, vin, nl);
+    const Moments mo = runMoments<kLanesA, true>(P, T, first, cs, pos, vin, nl);
     const u32 i = first + threadIdx.x;
     if (i >= P.pEnd) return;
     const float4 pi = pos[i];
@@ -231,12 +249,12 @@ __global__ __launch_bounds__(256) void k_solveB(const KP* __restrict__ kps, Work
     sceneMats(T, mats, scene, &curScene);
     const KP& P = kps[scene];
     __shared__ float lds[SolveBAcc::kF][256];
-    const SolveBAcc a = runPasses<SolveBAcc>(lds, first, P.pEnd, [&](u32 i, u32 q, SolveBAcc& acc) {
+    const SolveBAcc a = runPasses<kLanesB, SolveBAcc>(lds, first, P.pEnd, [&](u32 i, u32 q, SolveBAcc& acc) {
       const float4 pi = pos[i];
       if (!((T.solidMask >> matOfTag(pi.w)) & 1u)) return;
       const float4 vi4 = vin[i];
-      if (allSolid(T)) solveBGatherQ<true>(P, T, i, q, pi, vi4, pos, vin, Sp, cs, nl, acc);
-      else solveBGatherQ<false>(P, T, i, q, pi, vi4, pos, vin, Sp, cs, nl, acc);
+      if (allSolid(T)) solveBGatherQ<kLanesB, true>(P, T, i, q, pi, vi4, pos, vin, Sp, cs, nl, acc);
+      else solveBGatherQ<kLanesB, false>(P, T, i, q, pi, vi4, pos, vin, Sp, cs, nl, acc);
     });
     const u32 i = first + threadIdx.x;
     if (i >= P.pEnd) return;
@@ -356,14 +374,14 @@ __global__ __launch_bounds__(256) void k_xsph(const KP* __restrict__ kps, WorkTa
     sceneMats(T, mats, scene, &curScene);
     const KP& P = kps[scene];
     __shared__ float lds[F3Acc::kF][256];
-    const F3Acc a = runPasses<F3Acc>(lds, first, P.pEnd, [&](u32 i, u32 q, F3Acc& acc) {
+    const F3Acc a = runPasses<kLanesX, F3Acc>(lds, first, P.pEnd, [&](u32 i, u32 q, F3Acc& acc) {
       const float4 pi = pos[i];
       const f3 vi = xyz(vin[i]);
       const bool iSolid = solidTag(T.solidMask, pi.w);
       // kAll: no water in the scene, every neighbour shares the particle's phase
       auto gather = [&](auto all) {
         constexpr bool kAll = decltype(all)::value;
-        forSupport(P, cs, pos, nl, i, xyz(pi), q, [&](u32 b) { return PV{gat(pos, b), gat(vin, b)}; },
+        forSupport<kLanesX>(P, cs, pos, nl, i, xyz(pi), q, [&](u32 b) { return PV{gat(pos, b), gat(vin, b)}; },
                     [&](u32 b, const PV& r, f3 dxv, float r2) {
                       if (kAll || solidTag(T.solidMask, r.p.w) == iSolid)
                         acc.v = acc.v + r.v.w * (xyz(r.v) - vi) * kernelWfast(P, r2);
@@ -395,7 +413,7 @@ __global__ __launch_bounds__(256) void k_updateF(const KP* __restrict__ kps, Wor
   forWork<kPersist>(W, [&](u32 scene, u32 first) {
     sceneMats(T, mats, scene, &curScene);
     const KP& P = kps[scene];
-    const Moments mo = runMoments<false>(P, T, first, cs, pos, vin, nl);
+    const Moments mo = runMoments<kLanesF, false>(P, T, first, cs, pos, vin, nl);
     const u32 i = first + threadIdx.x;
     if (i >= P.pEnd) return;
     const float4 pi = pos[i];
@@ -462,7 +480,7 @@ __global__ __launch_bounds__(256) void k_features(const KP* __restrict__ kps, Wo
   forWork<kPersist>(W, [&](u32 scene, u32 first) {
     sceneMats(T, mats, scene, &curScene);
     const KP& P = kps[scene];
-    const Moments mo = runMoments<true>(P, T, first, cs, pos, vin, nl);
+    const Moments mo = runMoments<kLanesA, true>(P, T, first, cs, pos, vin, nl);
     const u32 i = first + threadIdx.x;
     if (i >= P.pEnd) return;
     float row[16];

@@ -18,8 +18,10 @@ never the 1M column itself).  With --no-residual, `value` is the fixed
 
 Also reported: the fixed 20-iteration throughput replayed from a HIP graph under
 the autotuned launch shape; per-kernel HIP-event times of an eager run; the HBM
-roofline of the dominant kernel (solveA, with its HBM traffic from the committed
-counter profile); the 64-scene batched ensemble; the mixed water + sand scene;
+roofline of the kernel with the largest share of the substep (solveB) and of
+solveA beside it, with HBM traffic, VALU and address-unit busy fractions from
+rocprofv3 counter passes of this build's native driver run at the end; the
+64-scene batched ensemble; the mixed water + sand scene;
 the von Mises poke to residual; the prototype's 2D fp64 40x80 column
 beside its numpy oracle; and the f32 C++ oracle's substep on the host cores (a
 reported baseline).
@@ -113,6 +115,113 @@ def learned_vs_plain(sim, tol, cap, substeps, params, train, before=None, warmup
                 note="learned iterations count the learned first iteration", training=train)
 
 
+def host_cpu():
+    """The host's CPU model and core counts (a reported baseline's provenance)."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return dict(model=model, logical_cpus=os.cpu_count(), usable_cpus=usable)
+
+
+def _pmc_pass(counters, tag, regex, out_root, bench_args):
+    """One rocprofv3 --pmc pass over the native C-ABI driver; returns per kernel
+    (last 10 dispatches) the mean of each counter summed over the chip, and the
+    kernels' mean dispatch durations in us.  None when the pass fails."""
+    import glob
+    import sqlite3
+    import subprocess
+    exe = os.path.join(REPO, "xpbi-webgpu_amd", "xpbi_bench")
+    d = os.path.join(out_root, tag)
+    cmd = ["rocprofv3", "--pmc"] + counters + ["--kernel-include-regex", regex, "-d", d, "-o", "run", "--", exe] + \
+        bench_args
+    try:
+        r = subprocess.run(cmd, cwd=out_root, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=120,
+                           env=dict(os.environ, TMPDIR=out_root))
+    except (OSError, subprocess.TimeoutExpired):
+        return None
+    dbs = glob.glob(os.path.join(d, "**", "*.db"), recursive=True)
+    if r.returncode != 0 or not dbs:
+        return None
+    con = sqlite3.connect(dbs[0])
+    cur = con.cursor()
+    tabs = {row[0] for row in cur.execute("select name from sqlite_master where type='table'")}
+    T = lambda pre: [t for t in tabs if t.startswith(pre)][0]
+    pmc, info, disp, sym = T("rocpd_pmc_event"), T("rocpd_info_pmc"), T("rocpd_kernel_dispatch"), \
+        T("rocpd_info_kernel_symbol")
+    symcols = [c[1] for c in cur.execute(f"pragma table_info({sym})")]
+    infocols = [c[1] for c in cur.execute(f"pragma table_info({info})")]
+    kcol = "kernel_name" if "kernel_name" in symcols else symcols[1]
+    ncol = "name" if "name" in infocols else infocols[1]
+    rows = cur.execute(f"select d.id, s.{kcol}, i.{ncol}, p.value, d.start, d.end from {pmc} p join {info} i "
+                       f"on p.pmc_id = i.id join {disp} d on p.event_id = d.event_id join {sym} s "
+                       f"on d.kernel_id = s.id").fetchall()
+    con.close()
+    import re
+    per, dur, name = {}, {}, {}
+    for did, kn, cn, v, t0, t1 in rows:
+        m = re.match(r"_ZN4xpbi(\d+)", kn)
+        k = kn[m.end():m.end() + int(m.group(1))] if m else kn.split("(")[0]
+        per.setdefault(did, {}).setdefault(cn, 0.0)
+        per[did][cn] += v
+        dur[did], name[did] = (t1 - t0) / 1e3, k
+    out = {}
+    for k in set(name.values()):
+        ids = sorted(i for i in per if name[i] == k)[-10:]
+        agg = {c: sum(per[i].get(c, 0.0) for i in ids) / len(ids) for c in per[ids[0]]}
+        agg["us"] = sum(dur[i] for i in ids) / len(ids)
+        out[k] = agg
+    return out
+
+
+def counter_passes(N):
+    """HBM traffic (FETCH_SIZE, WRITE_SIZE, calibrated on the library's copy kernel of
+    a known size), VALU issue and texture-address-unit busy fractions of the
+    gather kernels, from rocprofv3 --pmc passes of this build's native driver on the
+    same 1,024,000-particle column (20 substeps from rest, then one counted
+    substep; runs of that length profile cleanly, runs of hundreds of substeps have
+    ended in a segmentation fault inside the profiler's dispatch interception)."""
+    import shutil
+    import tempfile
+    if shutil.which("rocprofv3") is None:
+        return None
+    root = tempfile.mkdtemp(prefix="xpbi_pmc_")
+    args = ["--settle", "20", "--steps", "1", "--copy-mib", "256"]
+    regex = "k_solve|k_computeL|k_xsph|k_updateF|k_copy"
+    f = _pmc_pass(["FETCH_SIZE"], "fetch", regex, root, args)
+    w = _pmc_pass(["WRITE_SIZE"], "write", regex, root, args)
+    q = _pmc_pass(["SQ_INSTS_VALU", "SQ_WAVES", "SQ_INSTS_VMEM_RD", "TA_TA_BUSY", "TA_FLAT_READ_WAVEFRONTS",
+                   "GRBM_GUI_ACTIVE"], "sq", regex, root, args)
+    shutil.rmtree(root, ignore_errors=True)
+    if not f or not w or "k_copy" not in f or "k_copy" not in w:
+        return None
+    copy_bytes = 256 << 20
+    fs = copy_bytes / (f["k_copy"]["FETCH_SIZE"] * 1024.0)  # counter KiB -> bytes read by the copy
+    ws = copy_bytes / (w["k_copy"]["WRITE_SIZE"] * 1024.0)
+    out = dict(fetch_scale=round(fs, 4), write_scale=round(ws, 4), kernels={})
+    for k in ("k_solveB", "k_solveA", "k_computeL", "k_xsph", "k_updateF"):
+        if k not in f or k not in w:
+            continue
+        t = f[k]["FETCH_SIZE"] * 1024.0 * fs + w[k]["WRITE_SIZE"] * 1024.0 * ws
+        e = dict(traffic_bytes=round(t), traffic_bytes_per_particle=round(t / N, 1))
+        if q and k in q:
+            c = q[k]
+            cyc = c["GRBM_GUI_ACTIVE"] / 8.0  # per XCD: the kernel's cycles
+            e["valu_busy"] = round(c["SQ_INSTS_VALU"] * 4.0 / 1024.0 / cyc, 3)  # 1024 SIMDs, 4 cycles per wave64 op
+            e["ta_busy"] = round(c["TA_TA_BUSY"] / 256.0 / cyc, 3)
+            e["ta_cycles_per_load"] = round(c["TA_TA_BUSY"] / max(c["TA_FLAT_READ_WAVEFRONTS"], 1.0), 1)
+        out["kernels"][k] = e
+    return out
+
+
 def cpu_baseline(sim, scene, iterations, threads):
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle_py
@@ -127,9 +236,11 @@ def cpu_baseline(sim, scene, iterations, threads):
     t0 = time.perf_counter()
     oracle_py.substep(st, sim.params, mats, iterations, scene["hasWater"], nthreads=threads)
     dt = time.perf_counter() - t0
+    cpu = host_cpu()
     return dict(value=sim.N / dt, unit="particle-substeps/s", cores=threads, kind="port",
                 sample=f"1 substep ({iterations} Jacobi iterations) of the settled {sim.N}-particle column, "
-                       f"f32 C++ oracle, OpenMP {threads} threads, {dt:.1f} s")
+                       f"f32 C++ oracle, OpenMP {threads} threads, {dt:.1f} s",
+                cpu_model=cpu["model"], host_logical_cpus=cpu["logical_cpus"], host_usable_cpus=cpu["usable_cpus"])
 
 
 def column_2d(with_cpu):
@@ -159,11 +270,14 @@ def column_2d(with_cpu):
     return out
 
 
-def mixed_scene(settle, steps):
-    """Dam breach: 600k PBF water beside a 400k DP sand wall (phi 35), 10 iterations."""
+def mixed_scene(settle, steps, s_corr=None):
+    """Dam breach: 600k PBF water beside a 400k DP sand wall (phi 35), 10 iterations;
+    s_corr: override of the configured k = 0.1."""
     import xpbi
     from xpbi import scenes
     sc = scenes.dam_breach_1m()
+    if s_corr is not None:
+        sc["sCorrK"] = s_corr
     I = sc["iterations"]
     sim = xpbi.Simulation(sc, device=0)
     sim.writeParams(xpbi.params_from_scene(sc))
@@ -180,7 +294,17 @@ def mixed_scene(settle, steps):
     g = ("computeL", "solveA", "solveB", "pbfA", "pbfB", "xsph", "updateF")
     g_ms = sum(per[k][0] for k in g if k in per) / 4
     g_bytes = (264 + 348 * I) * N  # compulsory bytes of the gather kernels with water (upper bound)
+    vel = sim.read("VEL")[:, :3]
+    import numpy as np
+    speed = np.linalg.norm(vel, axis=1)
+    vmax = 0.45 * 2 * sc["h"] / sc["dt"]  # integrate's velocity clamp (simkernels.js:535)
+    at_clamp = float((speed > 0.99 * vmax).mean())
+    state = (f"burst: {100 * at_clamp:.0f} % of particles at the {vmax:.0f} m/s velocity clamp (s_corr k = "
+             f"{sc['sCorrK']:g}); PBF cost here is that of clumped cells, not of a stable dam breach"
+             if at_clamp > 0.05 else f"no burst: {100 * at_clamp:.1f} % at the velocity clamp (s_corr k = "
+             f"{sc['sCorrK']:g}), median speed {float(np.median(speed)):.2f} m/s")
     out = dict(particles=N, water=int((sc["matIds"] == 1).sum()), sand=int((sc["matIds"] == 0).sum()),
+               s_corr_k=sc["sCorrK"], state=state,
                iterations=I, settle_substeps=settle, ms_per_substep_graph=round(ms, 4),
                particle_substeps_per_s=round(N / (ms / 1e3), 1), kernels_us=kern,
                gather_kernels_GBs=round(g_bytes / (g_ms / 1e3) / 1e9, 1),
@@ -270,6 +394,7 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-configs", action="store_true", help="skip the mixed (c3), poke (c5) and 2D (c1) scenes")
     ap.add_argument("--only-2d", action="store_true", help="run only the 2D 40x80 fp64 column; print its object")
+    ap.add_argument("--no-counters", action="store_true", help="skip the rocprofv3 counter passes")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: a simulation runs on one device and the engine has no "
@@ -310,17 +435,20 @@ def main():
         if name in ALG_BYTES:
             ent["alg_GBs"] = round(ALG_BYTES[name] * N / (avg / 1e3) / 1e9, 1)
         kern[name] = ent
-    a_ms = per["solveA"][0] / per["solveA"][1]
-    achieved = ALG_BYTES["solveA"] * N / (a_ms / 1e3) / 1e9
-    traffic = None
-    tf = os.path.join(REPO, "profiles", "solveA_headline_traffic.txt")
-    if os.path.exists(tf):
-        for line in open(tf):
-            if line.startswith("traffic_bytes_per_launch="):
-                traffic = float(line.split("=")[1])
-    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic, kernel="solveA",
-                    alg_bytes_per_launch=ALG_BYTES["solveA"] * N)
+    # the roofline kernel is the one with the largest share of the substep (solveB
+    # on this workload); solveA, the other per-iteration sweep, beside it
+    shares = {k: v[0] for k, v in per.items() if k in ALG_BYTES}
+    top = max(shares, key=shares.get)
+
+    def roof(k):
+        k_ms = per[k][0] / per[k][1]
+        ach = ALG_BYTES[k] * N / (k_ms / 1e3) / 1e9
+        return dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(ach / HBM_PEAK_GBS, 4), traffic=None, kernel=k,
+                    share=round(per[k][0] / total_ms, 4), alg_bytes_per_particle=ALG_BYTES[k],
+                    alg_bytes_per_launch=ALG_BYTES[k] * N, avg_us=round(k_ms * 1e3, 2))
+    roofline = roof(top)
+    roofline_other = roof("solveA" if top != "solveA" else "solveB")
     g_ms = sum(per[k][0] for k in GATHER if k in per)
     g_bytes = (264 + 260 * I) * N * detail_steps
     gather = dict(achieved_GBs=round(g_bytes / (g_ms / 1e3) / 1e9, 1),
@@ -348,16 +476,34 @@ def main():
         cpu = cpu_baseline(sim, scene, I, threads)
     sim.close()
 
+    counters = None
+    if not args.no_counters:
+        print("[bench] counter passes (rocprofv3 --pmc) of the native driver", file=sys.stderr, flush=True)
+        counters = counter_passes(N)
+        if counters:
+            for r in (roofline, roofline_other):
+                c = counters["kernels"].get("k_" + r["kernel"])
+                if c:
+                    r["traffic"] = c["traffic_bytes"]
+                    r["traffic_source"] = ("rocprofv3 FETCH_SIZE + WRITE_SIZE of this build's native driver on the "
+                                           "same column (20 substeps from rest, one counted substep), calibrated on "
+                                           "a 256 MiB copy")
+                    for key in ("valu_busy", "ta_busy", "ta_cycles_per_load"):
+                        if key in c:
+                            r[key] = c[key]
+
     ens = None
     if not args.no_ensemble:
         print("[bench] 64-scene batched ensemble", file=sys.stderr, flush=True)
         ens = ensemble(I, 300, 20)
-    mixed = poke = c1 = None
+    mixed = mixed_calm = poke = c1 = None
     if not args.no_configs:
         print("[bench] 2D 40x80 column on the fp64 engine", file=sys.stderr, flush=True)
         c1 = column_2d(not args.no_cpu)
         print("[bench] mixed water + sand scene", file=sys.stderr, flush=True)
         mixed = mixed_scene(60, 10)
+        print("[bench] mixed water + sand scene without s_corr", file=sys.stderr, flush=True)
+        mixed_calm = mixed_scene(60, 10, s_corr=0.0)
         if net is not None:
             print("[bench] von Mises block + poke: plain vs learned", file=sys.stderr, flush=True)
             poke = poke_scene(net, train, 200, 20, args.residual_cap)
@@ -384,9 +530,11 @@ def main():
         "learned_vs_plain_time_ratio": resid["time_ratio"] if resid else None,
         "fixed_iterations": fixed,
         "eager_ms_per_step": round(total_ms / detail_steps, 4),
-        "roofline": roofline, "gather_kernels_roofline": gather, "kernels": kern,
+        "roofline": roofline, "roofline_solveA" if roofline_other["kernel"] == "solveA" else "roofline_solveB":
+        roofline_other, "gather_kernels_roofline": gather, "kernels": kern, "counters": counters,
         "stream_copy_GBs": round(copy_GBs, 1),
         "jacobi_to_residual": resid, "batched_ensemble": ens, "mixed_water_sand": mixed,
+        "mixed_water_sand_no_s_corr": mixed_calm,
         "vm_block_poke_to_residual": poke, "column_2d": c1, "cpu_baseline": cpu,
     }
     print(json.dumps(out))

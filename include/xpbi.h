@@ -344,6 +344,31 @@ int xpbi_mlp_adam_step(xpbi_mlp* mlp, const float* grad, float lr);
  * (seeded); epoch_loss (epochs floats, may be NULL) gets each epoch's mean loss. */
 int xpbi_mlp_train(xpbi_mlp* mlp, int epochs, int batch, float lr, uint64_t seed, float* epoch_loss);
 
+/* ---- Allocation-free substep.  No counterpart in the reference; for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  One substep of xpbi_substep
+ * (web/js/sim.js:220-272; fixed iteration count, no warm start) of a single scene
+ * whose state lives in the caller's device buffers, in the reference layouts of
+ * xpbi_read: pos (16N: x, y, z, tag), vel (16N: velA), F (48N: three vec4 columns)
+ * and aux (16N), each 16-byte aligned; the substep reads them and overwrites them
+ * with its result, the particles in cell order as the reference's reorder leaves
+ * them.  p is the 256-byte Params record (p->N particles, grid words as
+ * xpbi_create computes them); mats holds n <= 8 64-byte records, mass filled
+ * (xpbi_write_materials).  Work is enqueued on `stream` (a hipStream_t; NULL: the
+ * null stream) of the current device, and the call returns without waiting.
+ * Every intermediate (sorted copies, cell table, sort keys, neighbour list, L,
+ * Sp) lives in a static device array of the library of
+ * xpbi_noalloc_scratch_bytes() bytes, one per device, held until the process
+ * ends; the call allocates nothing.  Limits: N <= XPBI_NOALLOC_MAX_PARTICLES and
+ * numCells <= XPBI_NOALLOC_MAX_CELLS, else XPBI_ERR_ARG (message from
+ * xpbi_last_error(NULL)).  All calls on one device share that array: calls that
+ * overlap on one device (other streams, other threads) race. */
+#define XPBI_NOALLOC_MAX_PARTICLES 65536u
+#define XPBI_NOALLOC_MAX_CELLS 1048576u
+int xpbi_substep_noalloc(const xpbi_params* p, const xpbi_material* mats, int n, void* pos, void* vel, void* F,
+                         void* aux, int iterations, int has_water, void* stream);
+size_t xpbi_noalloc_scratch_bytes(void);
+
 /* ---- 2D fp64 engine: the algorithm of prototype/xpbi2d.py:123-241 ---------
  * The prototype is a script (no API): a sand column whose state is x, v, F
  * (xpbi2d.py:52-58) advanced by the main loop (xpbi2d.py:123-241) with

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 #include "xpbi_internal.hpp"
@@ -960,6 +961,199 @@ int xpbi_autotune(xpbi_sim* h, int iterations, int has_water, int substeps, int*
   if (r) return r;
   if (best_blocks) *best_blocks = best;
   if (best_ms) *best_ms = bestMs;
+  return XPBI_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- allocation-free substep
+// xpbi_substep_noalloc: one substep of a single scene on the caller's buffers,
+// every intermediate in `scratch`, a static device array of this library (each
+// device holds its own copy until the process ends).  Host code never uses the
+// array's name as a device address: its address on each device comes from
+// hipGetSymbolAddress, taken once per device with that device current.
+namespace {
+
+constexpr u32 kNaMaxN = XPBI_NOALLOC_MAX_PARTICLES;
+constexpr u32 kNaMaxC = XPBI_NOALLOC_MAX_CELLS;
+constexpr u32 kNaSlots = 64;  // support-neighbour list slots (more neighbours: the row walk)
+constexpr size_t kNaAlign = 256;
+constexpr size_t naUp(size_t b) { return (b + kNaAlign - 1) / kNaAlign * kNaAlign; }
+// the scratch layout: per-particle arrays for kNaMaxN particles, per-cell arrays
+// for kNaMaxC cells, then the scan/radix partials and the scene's tables
+constexpr size_t kNaPerParticle = 2 * (16 + 16 + 16 + 4 + 16 + 4)  // the two particle sets
+                                  + 2 * 16                          // velocities
+                                  + 2 * 4 + 2 * 4                   // sort keys and values
+                                  + 16 + 16 + 4                     // L
+                                  + 16 + 16 + 4 + 16                // Sp, selfDv
+                                  + 2 * kNaSlots + 8                // neighbour list and heads
+                                  + 16;                             // predicted positions
+// per-particle and per-cell arrays at the maximum sizes, plus 1 MiB for the
+// scan and radix partials, the tables and the alignment of each array
+constexpr size_t kNaScratchBytes = kNaPerParticle * (size_t)kNaMaxN + 8ull * kNaMaxC + (1ull << 20);
+
+__device__ __align__(256) unsigned char scratch[kNaScratchBytes];
+
+// the scratch address on each device (set once per device, under naLock)
+constexpr int kNaMaxDevices = 64;
+unsigned char* g_naPtr[kNaMaxDevices] = {};
+std::mutex g_naLock;
+
+struct NaTables {
+  KP kp;
+  DevMat mats[8];
+};
+
+// the scene's tables into scratch: its KP record, 8 materials and the work table
+// (one entry per run of 256 particles)
+__global__ __launch_bounds__(256) void k_na_tables(NaTables t, KP* __restrict__ kp, DevMat* __restrict__ mats,
+                                                   uint2* __restrict__ tab, u32 nb) {
+  if (threadIdx.x == 0) *kp = t.kp;
+  if (threadIdx.x < 8) mats[threadIdx.x] = t.mats[threadIdx.x];
+  for (u32 e = threadIdx.x; e < nb; e += 256u) tab[e] = make_uint2(0u, 256u * e);
+}
+
+// caller layouts (pos, vel, aux: float4; F: three vec4 columns) -> set 0 and
+// velocity 0; ids 0..N-1
+__global__ __launch_bounds__(256) void k_na_load(const float4* __restrict__ pos, const float4* __restrict__ vel,
+                                                 const float4* __restrict__ F, const float4* __restrict__ aux, u32 N,
+                                                 PSet d, float4* __restrict__ vd) {
+  const u32 i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= N) return;
+  d.pos[i] = pos[i];
+  vd[i] = vel[i];
+  d.aux[i] = aux[i];
+  const float4 c0 = F[3 * (size_t)i], c1 = F[3 * (size_t)i + 1], c2 = F[3 * (size_t)i + 2];
+  d.Fa[i] = make_float4(c0.x, c0.y, c0.z, c1.x);
+  d.Fb[i] = make_float4(c1.y, c1.z, c2.x, c2.y);
+  d.Fc[i] = c2.z;
+  d.id[i] = i;
+}
+
+// the substep's final set and velocity -> the caller's buffers (pad lanes of F 0)
+__global__ __launch_bounds__(256) void k_na_store(PSet s, const float4* __restrict__ vs, u32 N,
+                                                  float4* __restrict__ pos, float4* __restrict__ vel,
+                                                  float4* __restrict__ F, float4* __restrict__ aux) {
+  const u32 i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= N) return;
+  pos[i] = s.pos[i];
+  vel[i] = vs[i];
+  aux[i] = s.aux[i];
+  const float4 a = s.Fa[i], b = s.Fb[i];
+  F[3 * (size_t)i] = make_float4(a.x, a.y, a.z, 0.0f);
+  F[3 * (size_t)i + 1] = make_float4(a.w, b.x, b.y, 0.0f);
+  F[3 * (size_t)i + 2] = make_float4(b.z, b.w, s.Fc[i], 0.0f);
+}
+
+// the scratch address on the current device (taken on the first call there)
+int na_scratch(int dev, unsigned char** out) {
+  if (dev < 0 || dev >= kNaMaxDevices) return fail(nullptr, XPBI_ERR_ARG, "device ordinal out of range");
+  std::lock_guard<std::mutex> g(g_naLock);
+  if (!g_naPtr[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return hip_fail(nullptr, e, "hipGetSymbolAddress(scratch)");
+    g_naPtr[dev] = (unsigned char*)p;
+  }
+  *out = g_naPtr[dev];
+  return XPBI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t xpbi_noalloc_scratch_bytes(void) { return kNaScratchBytes; }
+
+int xpbi_substep_noalloc(const xpbi_params* p, const xpbi_material* mats, int n, void* pos, void* vel, void* F,
+                         void* aux, int iterations, int has_water, void* stream) {
+  if (!p || (n > 0 && !mats)) return fail(nullptr, XPBI_ERR_ARG, "params or materials is null");
+  if (n < 0 || n > 8) return fail(nullptr, XPBI_ERR_ARG, "material count must be 0..8");
+  const u32 N = p->N, C = p->numCells;
+  if (N > kNaMaxN) return fail(nullptr, XPBI_ERR_ARG, "more particles than the allocation-free substep holds");
+  if (C > kNaMaxC) return fail(nullptr, XPBI_ERR_ARG, "more cells than the allocation-free substep holds");
+  if (iterations < 0) return fail(nullptr, XPBI_ERR_ARG, "iterations < 0");
+  if (p->gridDims[0] < 1 || p->gridDims[1] < 1 || p->gridDims[2] < 1 ||
+      (u64)p->gridDims[0] * (u64)p->gridDims[1] * (u64)p->gridDims[2] != (u64)C)
+    return fail(nullptr, XPBI_ERR_ARG, "gridDims do not multiply to numCells");
+  if (!(p->dt > 0.0f) || !(p->h > 0.0f)) return fail(nullptr, XPBI_ERR_ARG, "dt and h must be > 0");
+  if (!(p->support > 0.0f) || p->support > p->cellSize)
+    return fail(nullptr, XPBI_ERR_ARG, "support must be in (0, cellSize]");
+  for (void* b : {pos, vel, F, aux})
+    if (N > 0 && (!b || ((uintptr_t)b & 15u))) return fail(nullptr, XPBI_ERR_ARG, "state buffers must be 16-byte aligned");
+  if (N == 0) return XPBI_OK;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(nullptr, e, "hipGetDevice");
+  unsigned char* base = nullptr;
+  int r = na_scratch(dev, &base);
+  if (r) return r;
+
+  // carve the scratch (the same arrays xpbi_create allocates, for this N and C)
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    void* q = base + off;
+    off += naUp(bytes);
+    return q;
+  };
+  Sim s;  // host bookkeeping only: every pointer below is into scratch
+  s.dev = dev;
+  s.stream = (hipStream_t)stream;
+  s.N = N;
+  s.C = C;
+  s.nb = (N + 255) / 256;
+  s.grid = s.nb;
+  s.persist = false;
+  for (int k = 0; k < 2; k++) {
+    s.set[k].pos = (float4*)take(16ull * N);
+    s.set[k].Fa = (float4*)take(16ull * N);
+    s.set[k].Fb = (float4*)take(16ull * N);
+    s.set[k].Fc = (float*)take(4ull * N);
+    s.set[k].aux = (float4*)take(16ull * N);
+    s.set[k].id = (u32*)take(4ull * N);
+    s.vel[k] = (float4*)take(16ull * N);
+    s.keys[k] = (u32*)take(4ull * N);
+    s.vals[k] = (u32*)take(4ull * N);
+  }
+  s.L = M3P{(float4*)take(16ull * N), (float4*)take(16ull * N), (float*)take(4ull * N)};
+  s.Sp = M3P{(float4*)take(16ull * N), (float4*)take(16ull * N), (float*)take(4ull * N)};
+  s.selfDv = (float4*)take(16ull * N);
+  s.nbrK = kNaSlots;
+  s.nbrLst = (uint16_t*)take(2ull * kNaSlots * N);
+  s.nbrHead = (uint2*)take(8ull * N);
+  s.xc = (float4*)take(16ull * N);
+  s.cellStart = (u32*)take(4ull * ((size_t)C + 1));
+  s.count = (u32*)take(4ull * C);
+  const size_t nr = radix_hist_needed(N), np = std::max(scan_partials_needed(C), scan_partials_needed(nr));
+  s.radixHist = (u32*)take(4ull * std::max<size_t>(nr, 1));
+  s.scanPart = (u32*)take(4ull * np);
+  s.dKP = (KP*)take(sizeof(KP));
+  s.dMats = (DevMat*)take(8 * sizeof(DevMat));
+  s.dTab = (uint2*)take(sizeof(uint2) * s.nb);
+  if (off > kNaScratchBytes) return fail(nullptr, XPBI_ERR_ARG, "scratch layout exceeds the scratch array");
+  s.perm = s.vals[0];
+  s.sortedKeys = s.keys[0];
+
+  Scene sc;
+  sc.N = N;
+  sc.C = C;
+  sc.params = *p;
+  build_kp(sc);
+  NaTables t;
+  t.kp = sc.kp;
+  std::memset(t.mats, 0, sizeof(t.mats));
+  for (int k = 0; k < n; k++) std::memcpy(&t.mats[k], &mats[k], sizeof(DevMat));
+  const u32 g = (N + 255) / 256;
+  hipLaunchKernelGGL(k_na_tables, dim3(1), dim3(256), 0, s.stream, t, s.dKP, s.dMats, s.dTab, s.nb);
+  hipLaunchKernelGGL(k_na_load, dim3(g), dim3(256), 0, s.stream, (const float4*)pos, (const float4*)vel,
+                     (const float4*)F, (const float4*)aux, N, s.set[0], s.vel[0]);
+  launch_sort(&s);
+  launch_reorder(&s);
+  launch_solve_stages(&s, iterations, has_water != 0, XPBI_STAGE_FULL);
+  hipLaunchKernelGGL(k_na_store, dim3(g), dim3(256), 0, s.stream, s.set[s.cur], (const float4*)s.vel[s.vcur], N,
+                     (float4*)pos, (float4*)vel, (float4*)F, (float4*)aux);
+  e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(nullptr, e, "xpbi_substep_noalloc");
   return XPBI_OK;
 }
 

@@ -112,6 +112,9 @@ SIGNATURES = {
     "xpbi_set_warmstart": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t]),
     "xpbi_collect_warmstart_pairs": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                                C.POINTER(C.c_float), C.c_size_t]),
+    "xpbi_substep_noalloc": (C.c_int, [C.POINTER(Params), C.POINTER(Material), C.c_int, _P, _P, _P, _P, C.c_int,
+                                       C.c_int, _P]),
+    "xpbi_noalloc_scratch_bytes": (C.c_size_t, []),
     "xpbi_mlp_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
     "xpbi_mlp_destroy": (None, [_P]),
     "xpbi_mlp_last_error": (C.c_char_p, [_P]),

@@ -70,15 +70,38 @@ __device__ __forceinline__ void forRowsG(const KP& P, const u32* __restrict__ cs
   });
 }
 
+// forRowsG with four lanes per particle: lane q takes the candidates beg + q,
+// beg + q + 4, ... of each row span, so that a wave-wide load reads four
+// consecutive candidates of each of 16 particles; the four loads of a step are
+// issued together.  use(b, record) in the lane's candidate order.
+template <class Load, class Use>
+__device__ __forceinline__ void forRowsQ(const KP& P, const u32* __restrict__ cs, i3 cc, u32 q, Load&& load,
+                                         Use&& use) {
+  forRowSpans(P, cs, cc, [&](u32 beg, u32 end) {
+    for (u32 b0 = beg + q; b0 < end; b0 += 16u) {
+      decltype(load(0u)) d[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        if (b0 + 4u * (u32)j < end) d[j] = load(b0 + 4u * (u32)j);
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        if (b0 + 4u * (u32)j < end) use(b0 + 4u * (u32)j, d[j]);
+    }
+  });
+}
+
 // Support-neighbour list.  Entry k of particle i is lst[k * N + i], the u16
 // offset b - head[i].x of neighbour b from the particle's first candidate
 // (cellStart at the stencil's lowest cell (x-1, y-1, z-1), clamped to the grid):
 // the candidates of a particle span two z-layers of cells and a little more.
 // head[i].y is the neighbour count; head[i].y > K (more than K neighbours, or
 // kNbrFlag set for an offset past 16 bits) makes the particle's sweeps walk the
-// cell rows.
+// cell rows.  Bit kNbrWater (not part of the count) is set for a solid particle
+// some candidate of whose 9 row spans may be water (computeL: the candidates'
+// tags span a water material); it is clear for water particles.
 typedef uint16_t u16;
 constexpr u32 kNbrFlag = 0x80000000u;
+constexpr u32 kNbrWater = 0x40000000u;
 constexpr u32 kNbrOffMax = 0xffffu;
 struct NLc {
   const uint2* __restrict__ head;
@@ -162,7 +185,7 @@ __device__ __forceinline__ void forSupport(const KP& P, const u32* __restrict__ 
   static_assert(kL == 4u || kL == 8u, "four or eight lanes per particle");
   constexpr int kE = 16 / (int)kL;  // the lane's slots per group
   const uint2 hd = nl.head[i];
-  const u32 n = hd.y;
+  const u32 n = hd.y & ~kNbrWater;
   const bool rows = n > nl.K;
   // list: the lane's word of group g is 8-byte word (g N + i) 4 + q mod 4; the
   // next group's word is prefetched

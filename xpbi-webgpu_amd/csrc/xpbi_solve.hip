@@ -41,7 +41,8 @@ constexpr u32 kLanesF = XPBI_LANES_F;  // updateF
 // absolute tolerance 0.2 runs in FP64.  The same candidate walk records the
 // support-neighbour list of the substep.
 template <bool kPersist>
-__global__ __launch_bounds__(256) void k_computeL(const KP* __restrict__ kps, WorkTab W, const float4* __restrict__ pos, const float4* __restrict__ vin,
+__global__ __launch_bounds__(256) void k_computeL(const KP* __restrict__ kps, WorkTab W, const DevMat* __restrict__ mats,
+                                                  const float4* __restrict__ pos, const float4* __restrict__ vin,
                                                   const u32* __restrict__ cs, M3P L, u16* __restrict__ nLst,
                                                   uint2* __restrict__ nHead, u32 K, u32 NT) {
   // each thread's first kStage list entries, written out slot by slot at the
@@ -49,11 +50,20 @@ __global__ __launch_bounds__(256) void k_computeL(const KP* __restrict__ kps, Wo
   // walk land in different slots per lane)
   constexpr u32 kStage = 64;
   __shared__ u16 stage[kStage][256];
+  __shared__ MatLds T;
+  u32 curScene = ~0u;
   const u32 tid = threadIdx.x;
   forWork<kPersist>(W, [&](u32 scene, u32 first) {
+    sceneMats(T, mats, scene, &curScene);
     const KP& P = kps[scene];
     const u32 i = first + tid;
     if (i >= P.pEnd) return;
+    // kNbrWater for a solid particle of a scene with water: whether the range of
+    // its candidates' tags spans a water material (a superset test: a set flag
+    // only makes pbfB walk the candidates, which finds nothing to add)
+    const u32 waterMask = ~T.solidMask & 0xFFu;
+    const bool wantWater = waterMask != 0u && ((T.solidMask >> matOfTag(pos[i].w)) & 1u);
+    float tmin = 3.0e38f, tmax = -3.0e38f;
     f3 xi = xyz(pos[i]);
     f33 A = mzero<float>();
     u32 n = 0, flag = 0;
@@ -89,6 +99,10 @@ __global__ __launch_bounds__(256) void k_computeL(const KP* __restrict__ kps, Wo
             d[j] = xi - xyz(pb[j]);
             r2[j] = dist2(d[j]);
             ok[j] = (b0 + j < end) && (b0 + j != i) && (r2[j] < P.supp2);
+            if (wantWater && b0 + j < end) {
+              tmin = fminf(tmin, pb[j].w);
+              tmax = fmaxf(tmax, pb[j].w);
+            }
           }
           float Vb[8];
 #pragma unroll
@@ -112,6 +126,14 @@ __global__ __launch_bounds__(256) void k_computeL(const KP* __restrict__ kps, Wo
     }
     // the particle's own offset must fit too (the sweeps' batch tails gather i)
     flag |= (i - base) > kNbrOffMax ? kNbrFlag : 0u;
+    if (wantWater && tmax >= tmin) {
+      if (!(tmin >= 0.0f) || !(tmax < 8.0f)) {
+        flag |= kNbrWater;  // tags outside [0, 8): no range test, walk
+      } else {
+        const u32 lo = (u32)tmin, hi = (u32)tmax;
+        if (waterMask & ((2u << hi) - (1u << lo))) flag |= kNbrWater;
+      }
+    }
     // the staged slots, one group of 16 (32 bytes, in the list's interleaved
     // order: position p holds slot 16 g + 4 (p mod 4) + p / 4) at a time
     const u32 ns = min(n, min(kStage, K));  // the list may hold fewer slots than the stage
@@ -272,92 +294,117 @@ struct XL {
   float4 x;
   float l;
 };
-// simkernels.js:407-439
+// pbfA's sums over the candidates: density, sum of gradients, sum of their squares
+struct PbfAAcc {
+  static constexpr int kF = 5;
+  float rho;
+  f3 gs;
+  float g2;
+  __device__ __forceinline__ static PbfAAcc zero() { return PbfAAcc{0.0f, splat3(0.0f), 0.0f}; }
+  template <u32 kL> __device__ __forceinline__ void laneSum() {
+    rho = xpbi::laneSum<kL>(rho);
+    gs = xpbi::laneSum<kL>(gs);
+    g2 = xpbi::laneSum<kL>(g2);
+  }
+  __device__ __forceinline__ void put(float (*lds)[256], u32 slot) const {
+    lds[0][slot] = rho;
+    putF3(lds, 1, slot, gs);
+    lds[4][slot] = g2;
+  }
+  __device__ __forceinline__ static PbfAAcc get(float (*lds)[256], u32 slot) {
+    return PbfAAcc{lds[0][slot], getF3(lds, 1, slot), lds[4][slot]};
+  }
+};
+// simkernels.js:407-439; the candidate walk with four lanes per particle
 template <bool kPersist>
 __global__ __launch_bounds__(256) void k_pbfA(const KP* __restrict__ kps, WorkTab W, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
                                               const float4* __restrict__ xc, float4* __restrict__ aux,
                                               const u32* __restrict__ cs, const SolveCtl* __restrict__ ctl, u32 k) {
   if (skipIteration(ctl, k)) return;
   __shared__ MatLds T;
+  __shared__ float lds[PbfAAcc::kF][256];
   u32 curScene = ~0u;
   forWork<kPersist>(W, [&](u32 scene, u32 first) {
     sceneMats(T, mats, scene, &curScene);
     const KP& P = kps[scene];
+    const PbfAAcc a = runPasses<4u, PbfAAcc>(lds, first, P.pEnd, [&](u32 i, u32 q, PbfAAcc& acc) {
+      const float4 pi = pos[i];
+      const u32 mi = matOfTag(pi.w);
+      if ((T.solidMask >> mi) & 1u) return;
+      const float* mRho = T.mRho[mi];  // m_b / rho0 by the neighbour's material
+      const f3 xci = xyz(xc[i]);
+      // candidates' predicted positions (x + dt v, tag) from the iteration's solveB
+      forRowsQ(P, cs, cellCoord(P, xyz(pi)), q, [&](u32 b) { return gat(xc, b); }, [&](u32 b, const float4& xb) {
+        if (b != i) {
+          const f3 dxv = xci - xyz(xb);
+          if (dist2(dxv) < P.supp2) {
+            const u32 kb = matOfTag(xb.w);
+            const float r = len(dxv);
+            acc.rho = acc.rho + T.mass[kb] * kernelPoly6(P, r);
+            const f3 g = kernelSpikyGradR(P, dxv, r) * mRho[kb];
+            acc.gs = acc.gs + g;
+            acc.g2 = acc.g2 + dot(g, g);
+          }
+        }
+      });
+    });
     const u32 i = first + threadIdx.x;
     if (i >= P.pEnd) return;
-    const float4 pi = pos[i];
-    const u32 mi = matOfTag(pi.w);
+    const u32 mi = matOfTag(pos[i].w);
     if ((T.solidMask >> mi) & 1u) { aux[i].w = 0.0f; return; }
-    const float rho0 = T.m[mi].density;
-    const float* mRho = T.mRho[mi];  // m_b / rho0 by the neighbour's material
-    const f3 xn = xyz(pi);
-    const f3 xci = xyz(xc[i]);
-    float rho = T.mass[mi] * P.poly6_0;
-    f3 gradSum = splat3(0.0f);
-    float grad2 = 0.0f;
-    // candidates' predicted positions (x + dt v, tag) from the iteration's solveB
-    forRowsG<8>(P, cs, cellCoord(P, xn), [&](u32 b) { return gat(xc, b); }, [&](u32 b, const float4& xb) {
-      if (b != i) {
-        f3 dxv = xci - xyz(xb);
-        if (dot(dxv, dxv) < P.supp2) {
-          const u32 kb = matOfTag(xb.w);
-          float mb = T.mass[kb];
-          const float r = len(dxv);
-          rho = rho + mb * kernelPoly6(P, r);
-          f3 g = kernelSpikyGradR(P, dxv, r) * mRho[kb];
-          gradSum = gradSum + g;
-          grad2 = grad2 + dot(g, g);
-        }
-      }
-    });
-    float C = wmax(rho / rho0 - 1.0f, 0.0f);
-    aux[i].w = -C / (grad2 + dot(gradSum, gradSum) + P.pbfEps);
+    const float rho = T.mass[mi] * P.poly6_0 + a.rho;
+    const float C = wmax(rho / T.m[mi].density - 1.0f, 0.0f);
+    aux[i].w = -C / (a.g2 + dot(a.gs, a.gs) + P.pbfEps);
   });
 }
 
-// simkernels.js:441-477
+// simkernels.js:441-477; the candidate walk with four lanes per particle.  A
+// solid particle none of whose candidates can be water (computeL's kNbrWater
+// clear) gets no correction and walks nothing.
 template <bool kPersist>
 __global__ __launch_bounds__(256) void k_pbfB(const KP* __restrict__ kps, WorkTab W, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
                                               const float4* __restrict__ vin, const float4* __restrict__ xc,
                                               const float4* __restrict__ aux, const u32* __restrict__ cs,
-                                              float4* __restrict__ vout, const SolveCtl* __restrict__ ctl, u32 k) {
+                                              const uint2* __restrict__ head, float4* __restrict__ vout,
+                                              const SolveCtl* __restrict__ ctl, u32 k) {
   if (skipIteration(ctl, k)) return;
   __shared__ MatLds T;
+  __shared__ float lds[F3Acc::kF][256];
   u32 curScene = ~0u;
   forWork<kPersist>(W, [&](u32 scene, u32 first) {
     sceneMats(T, mats, scene, &curScene);
     const KP& P = kps[scene];
+    const F3Acc a = runPasses<4u, F3Acc>(lds, first, P.pEnd, [&](u32 i, u32 q, F3Acc& acc) {
+      const float4 pi = pos[i];
+      const u32 mi = matOfTag(pi.w);
+      const bool iAmWater = !((T.solidMask >> mi) & 1u);
+      if (!iAmWater && !(head[i].y & kNbrWater)) return;
+      const f3 xci = xyz(xc[i]);
+      const float li = aux[i].w;
+      forRowsQ(P, cs, cellCoord(P, xyz(pi)), q, [&](u32 b) { return XL{gat(xc, b), gatW(aux, b)}; },
+               [&](u32 b, const XL& r) {
+                 if (b != i) {
+                   const u32 mb = matOfTag(r.x.w);
+                   const bool bWater = !((T.solidMask >> mb) & 1u);
+                   if (iAmWater || bWater) {
+                     const f3 dxv = xci - xyz(r.x);
+                     if (dist2(dxv) < P.supp2) {
+                       const u32 wk = bWater ? mb : mi;
+                       const float rr = len(dxv);
+                       const float ratio = kernelPoly6(P, rr) * P.invW0;
+                       const float r2 = ratio * ratio;
+                       const float sCorr = -P.sCorrK * (r2 * r2);
+                       acc.v = acc.v + (li + r.l + sCorr) * kernelSpikyGradR(P, dxv, rr) * T.mRho[wk][wk];
+                     }
+                   }
+                 }
+               });
+    });
     const u32 i = first + threadIdx.x;
     if (i >= P.pEnd) return;
-    const float4 pi = pos[i];
-    const u32 mi = matOfTag(pi.w);
-    const f3 xn = xyz(pi);
     const float4 vi4 = vin[i];
-    const f3 xci = xyz(xc[i]);
-    f3 dp = splat3(0.0f);
-    const float li = aux[i].w;
-    const bool iAmWater = !((T.solidMask >> mi) & 1u);
-    forRowsG<8>(P, cs, cellCoord(P, xn), [&](u32 b) { return XL{gat(xc, b), gatW(aux, b)}; }, [&](u32 b, const XL& r) {
-      if (b != i) {
-        u32 mb = matOfTag(r.x.w);
-        bool bWater = !((T.solidMask >> mb) & 1u);
-        if (iAmWater || bWater) {
-          f3 dxv = xci - xyz(r.x);
-          if (dot(dxv, dxv) < P.supp2) {
-            u32 wk = bWater ? mb : mi;
-            float lb = r.l;
-            const float r = len(dxv);
-            float w = kernelPoly6(P, r);
-            float ratio = w * P.invW0;
-            float r2 = ratio * ratio;
-            float sCorr = -P.sCorrK * (r2 * r2);
-            dp = dp + (li + lb + sCorr) * kernelSpikyGradR(P, dxv, r) * T.mRho[wk][wk];
-          }
-        }
-      }
-    });
-    float scale = iAmWater ? 1.0f : 0.5f;
-    f3 v = xyz(vi4) + dp * (scale / P.dt);
+    const float scale = (T.solidMask >> matOfTag(pos[i].w)) & 1u ? 0.5f : 1.0f;
+    const f3 v = xyz(vi4) + a.v * (scale / P.dt);
     vout[i] = make_float4(v.x, v.y, v.z, vi4.w);
   });
 }
@@ -519,7 +566,7 @@ __global__ __launch_bounds__(256) void k_features(const KP* __restrict__ kps, Wo
         row[6] = m.lam / (m.lam + 2.0f * m.mu);
         row[7] = (float)m.kind / 6.0f;
         row[8] = strength;
-        row[9] = (float)(nl.head[i].y & ~kNbrFlag) / 40.0f;
+        row[9] = (float)(nl.head[i].y & ~(kNbrFlag | kNbrWater)) / 40.0f;
         row[10] = (d0 != 0.0f && lamPrev) ? wclamp(lamPrev[i] / d0, -8.0f, 8.0f) : 0.0f;
         row[11] = P.alphaT / den;
         row[12] = d0;
@@ -615,7 +662,8 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
   M3P F{S.Fa, S.Fb, S.Fc};
   hipStream_t st = s->stream;
   int vi = s->vcur;
-  XPBI_LAUNCH_W(k_computeL, dim3(gN), dim3(256), 0, st, P, W, (const float4*)S.pos, (const float4*)s->vel[vi],
+  XPBI_LAUNCH_W(k_computeL, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
+                     (const float4*)s->vel[vi],
                      (const u32*)s->cellStart, s->L, s->nbrLst, s->nbrHead, s->nbrK, s->N);
   const NLc nl{s->nbrHead, s->nbrLst, s->nbrK, s->N};
   s->prof_mark("computeL");
@@ -651,7 +699,8 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
         s->prof_mark("pbfA");
         XPBI_LAUNCH_W(k_pbfB, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
                            (const float4*)s->vel[vi], (const float4*)s->xc, (const float4*)S.aux,
-                           (const u32*)s->cellStart, s->vel[vi ^ 1], (const SolveCtl*)nullptr, 0u);
+                           (const u32*)s->cellStart, (const uint2*)s->nbrHead, s->vel[vi ^ 1],
+                           (const SolveCtl*)nullptr, 0u);
         s->prof_mark("pbfB");
         vi ^= 1;
       }
@@ -686,7 +735,8 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
       s->prof_mark("pbfA");
       XPBI_LAUNCH_W(k_pbfB, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
                          (const float4*)s->vel[vi], (const float4*)s->xc, (const float4*)S.aux,
-                         (const u32*)s->cellStart, s->vel[vi ^ 1], (const SolveCtl*)ctl, (u32)k);
+                         (const u32*)s->cellStart, (const uint2*)s->nbrHead, s->vel[vi ^ 1], (const SolveCtl*)ctl,
+                         (u32)k);
       s->prof_mark("pbfB");
       vi ^= 1;
     }

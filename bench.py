@@ -267,6 +267,24 @@ def column_2d(with_cpu):
         dt = time.perf_counter() - t0
         out["cpu_baseline"] = dict(value=round(N * 20 / dt, 1), unit="particle-substeps/s", cores=1, kind="port",
                                    sample=f"the first 20 of the 300 substeps, numpy float64 oracle, {dt:.1f} s")
+    # the prototype script's own scene (666 particles, its constants) on the engine,
+    # beside the script itself, timed when its golden vectors were recorded
+    pp, px = proto2d.prototype_params(), proto2d.prototype_positions()
+    sim = proto2d.Prototype2D(px, pp)
+    pms = sim.time(150)
+    sim.close()
+    proto = dict(workload="the prototype script's own 666-particle column and constants (dx 0.008, 8 iterations, "
+                          "dt 2e-4)", engine_ms_per_step=round(pms / 150, 4),
+                 engine_particle_steps_per_s=round(len(px) * 150 / (pms / 1e3), 1))
+    golden = os.path.join(REPO, "tests", "golden", "proto2d_golden.npz")
+    if os.path.exists(golden):
+        import numpy as np
+        ms_ref = float(np.load(golden)["ms_per_step"])
+        proto["reference_script"] = dict(
+            ms_per_step=round(ms_ref, 2), particle_steps_per_s=round(len(px) / (ms_ref / 1e3), 1), cores=1,
+            kind="reference", where="the unmodified prototype/xpbi2d.py (numpy + scipy) on the build container's CPU "
+                                    "(8 vCPU Xeon 2.7 GHz), steps 3-20, recorded with its golden vectors")
+    out["prototype_scene"] = proto
     return out
 
 

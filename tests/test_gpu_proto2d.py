@@ -98,3 +98,25 @@ def test_cooperative_grid_equals_single_workgroup():
         sim.close()
     for k in out[False]:
         assert np.array_equal(out[False][k], out[True][k]), k
+
+
+def test_engine_matches_the_reference_prototype_run():
+    """The 2D engine against the reference prototype script's own run: its initial
+    column and its states after steps 1, 2, 5, 10 and 20, recorded in the build
+    container (tests/golden/make_proto2d_golden.py), to 1e-11 relative."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "proto2d_golden.npz"))
+    x = host.prototype_positions()
+    assert np.array_equal(x, g["x0"]), "initial column differs from the script's"
+    sim = host.Prototype2D(x, host.prototype_params())
+    done = 0
+    for k in g["steps"]:
+        sim.step(int(k) - done)
+        done = int(k)
+        for f, key in (("X", "x"), ("V", "v"), ("F", "F")):
+            ref = g[f"{key}{k}"]
+            got = sim.read(f).reshape(ref.shape)
+            err = float(np.abs(got - ref).max() / np.abs(ref).max())
+            print(f"step {k} {f}: rel err {err:.2e}", flush=True)
+            assert err <= TOL, (k, f, err)
+    sim.close()

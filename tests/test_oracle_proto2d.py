@@ -2,8 +2,11 @@
 prototype/xpbi2d.py) by what the mathematics fixes: LAPACK's singular values,
 the kernel's normalisation and gradient, brute-force and k-d-tree pair sets,
 the return map's cone, the energy's derivative, and whole-substep invariants
-(a rest lattice stays at rest, free fall, momentum of the internal impulses)."""
+(a rest lattice stays at rest, free fall, momentum of the internal impulses);
+and by the reference prototype's own run (golden vectors recorded from the
+unmodified script)."""
 import math
+import os
 
 import numpy as np
 import pytest
@@ -11,6 +14,8 @@ from scipy.spatial import cKDTree
 
 import proto2d as o
 from xpbi import proto2d as host
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def _mats(seed):
@@ -171,3 +176,23 @@ def test_host_and_oracle_columns_agree():
     assert np.array_equal(host.column40x80_positions(), o.column40x80_state()["x"])
     assert np.array_equal(host.prototype_positions(), o.prototype_state()["x"])
     assert len(host.prototype_positions()) == 666 and len(host.column40x80_positions()) == 3200
+
+
+def test_oracle_matches_the_reference_prototype_run():
+    """The reference's own prototype script, run unmodified in the build container
+    (tests/golden/make_proto2d_golden.py), recorded its initial column and its state
+    after steps 1, 2, 5, 10 and 20 (tests/golden/proto2d_golden.npz).  The oracle,
+    from the same parameters and column, lands on the same states: positions,
+    velocities and F to 1e-12 relative (numpy's LAPACK SVD and kd-tree pair order
+    against the oracle's closed-form 2x2 SVD and sorted pairs)."""
+    g = np.load(os.path.join(GOLDEN, "proto2d_golden.npz"))
+    st = o.prototype_state()
+    assert np.array_equal(st["x"], g["x0"]), "initial column differs from the script's"
+    prm, done = o.prototype_params(), 0
+    for k in g["steps"]:
+        o.run(prm, st, int(k) - done)
+        done = int(k)
+        for f in ("x", "v", "F"):
+            ref = g[f"{f}{k}"]
+            err = np.abs(st[f] - ref).max() / np.abs(ref).max()
+            assert err <= 1e-12, (k, f, err)

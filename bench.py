@@ -20,8 +20,8 @@ Also reported: the fixed 20-iteration throughput replayed from a HIP graph under
 the autotuned launch shape; per-kernel HIP-event times of an eager run; the HBM
 roofline of the kernel with the largest share of the substep (solveB) and of
 solveA beside it, with HBM traffic, VALU and address-unit busy fractions from
-rocprofv3 counter passes of this build's native driver run at the end; the
-64-scene batched ensemble; the mixed water + sand scene;
+rocprofv3 counter passes over one substep of the settled column (loaded from a
+checkpoint written after settling) run at the end; the 64-scene batched ensemble; the mixed water + sand scene;
 the von Mises poke to residual; the prototype's 2D fp64 40x80 column
 beside its numpy oracle; and the f32 C++ oracle's substep on the host cores (a
 reported baseline).
@@ -132,94 +132,32 @@ def host_cpu():
     return dict(model=model, logical_cpus=os.cpu_count(), usable_cpus=usable)
 
 
-def _pmc_pass(counters, tag, regex, out_root, bench_args):
-    """One rocprofv3 --pmc pass over the native C-ABI driver; returns per kernel
-    (last 10 dispatches) the mean of each counter summed over the chip, and the
-    kernels' mean dispatch durations in us.  None when the pass fails."""
-    import glob
-    import sqlite3
-    import subprocess
-    exe = os.path.join(REPO, "xpbi-webgpu_amd", "xpbi_bench")
-    d = os.path.join(out_root, tag)
-    cmd = ["rocprofv3", "--pmc"] + counters + ["--kernel-include-regex", regex, "-d", d, "-o", "run", "--", exe] + \
-        bench_args
-    try:
-        r = subprocess.run(cmd, cwd=out_root, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=120,
-                           env=dict(os.environ, TMPDIR=out_root))
-    except (OSError, subprocess.TimeoutExpired):
-        return None
-    dbs = glob.glob(os.path.join(d, "**", "*.db"), recursive=True)
-    if r.returncode != 0 or not dbs:
-        return None
-    con = sqlite3.connect(dbs[0])
-    cur = con.cursor()
-    tabs = {row[0] for row in cur.execute("select name from sqlite_master where type='table'")}
-    T = lambda pre: [t for t in tabs if t.startswith(pre)][0]
-    pmc, info, disp, sym = T("rocpd_pmc_event"), T("rocpd_info_pmc"), T("rocpd_kernel_dispatch"), \
-        T("rocpd_info_kernel_symbol")
-    symcols = [c[1] for c in cur.execute(f"pragma table_info({sym})")]
-    infocols = [c[1] for c in cur.execute(f"pragma table_info({info})")]
-    kcol = "kernel_name" if "kernel_name" in symcols else symcols[1]
-    ncol = "name" if "name" in infocols else infocols[1]
-    rows = cur.execute(f"select d.id, s.{kcol}, i.{ncol}, p.value, d.start, d.end from {pmc} p join {info} i "
-                       f"on p.pmc_id = i.id join {disp} d on p.event_id = d.event_id join {sym} s "
-                       f"on d.kernel_id = s.id").fetchall()
-    con.close()
-    import re
-    per, dur, name = {}, {}, {}
-    for did, kn, cn, v, t0, t1 in rows:
-        m = re.match(r"_ZN4xpbi(\d+)", kn)
-        k = kn[m.end():m.end() + int(m.group(1))] if m else kn.split("(")[0]
-        per.setdefault(did, {}).setdefault(cn, 0.0)
-        per[did][cn] += v
-        dur[did], name[did] = (t1 - t0) / 1e3, k
-    out = {}
-    for k in set(name.values()):
-        ids = sorted(i for i in per if name[i] == k)[-10:]
-        agg = {c: sum(per[i].get(c, 0.0) for i in ids) / len(ids) for c in per[ids[0]]}
-        agg["us"] = sum(dur[i] for i in ids) / len(ids)
-        out[k] = agg
-    return out
-
-
-def counter_passes(N):
+def counter_passes(N, checkpoint_path):
     """HBM traffic (FETCH_SIZE, WRITE_SIZE, calibrated on the library's copy kernel of
-    a known size), VALU issue and texture-address-unit busy fractions of the
-    gather kernels, from rocprofv3 --pmc passes of this build's native driver on the
-    same 1,024,000-particle column (20 substeps from rest, then one counted
-    substep; runs of that length profile cleanly, runs of hundreds of substeps have
-    ended in a segmentation fault inside the profiler's dispatch interception)."""
+    a known size), VALU issue and texture-address-unit busy fractions and the cache
+    lines per wave-wide load of the gather kernels, from rocprofv3 --pmc passes of
+    one substep of the settled column: each pass loads the checkpoint written after
+    settling (xpbi.run_scene --load) and runs one counted substep.  Passes of a few
+    substeps profile cleanly; runs of hundreds of substeps under --pmc have ended in
+    a segmentation fault inside the profiler's dispatch interception."""
     import shutil
-    import tempfile
+    import subprocess
+    from xpbi import counters
     if shutil.which("rocprofv3") is None:
         return None
-    root = tempfile.mkdtemp(prefix="xpbi_pmc_")
-    args = ["--settle", "20", "--steps", "1", "--copy-mib", "256"]
-    regex = "k_solve|k_computeL|k_xsph|k_updateF|k_copy"
-    f = _pmc_pass(["FETCH_SIZE"], "fetch", regex, root, args)
-    w = _pmc_pass(["WRITE_SIZE"], "write", regex, root, args)
-    q = _pmc_pass(["SQ_INSTS_VALU", "SQ_WAVES", "SQ_INSTS_VMEM_RD", "TA_TA_BUSY", "TA_FLAT_READ_WAVEFRONTS",
-                   "GRBM_GUI_ACTIVE"], "sq", regex, root, args)
-    shutil.rmtree(root, ignore_errors=True)
-    if not f or not w or "k_copy" not in f or "k_copy" not in w:
+    prog = [sys.executable, "-m", "xpbi.run_scene", "sand_column_1m", "--load", checkpoint_path, "--substeps", "1",
+            "--copy-mib", "256", "--every", "0"]
+    try:
+        d, _ = counters.collect(prog, "k_solve|k_computeL|k_xsph|k_updateF", copy_bytes=256 << 20)
+    except (RuntimeError, OSError, IndexError, KeyError, ZeroDivisionError, subprocess.TimeoutExpired) as e:
+        print(f"[bench] counter passes failed: {str(e)[:300]}", file=sys.stderr, flush=True)
         return None
-    copy_bytes = 256 << 20
-    fs = copy_bytes / (f["k_copy"]["FETCH_SIZE"] * 1024.0)  # counter KiB -> bytes read by the copy
-    ws = copy_bytes / (w["k_copy"]["WRITE_SIZE"] * 1024.0)
-    out = dict(fetch_scale=round(fs, 4), write_scale=round(ws, 4), kernels={})
-    for k in ("k_solveB", "k_solveA", "k_computeL", "k_xsph", "k_updateF"):
-        if k not in f or k not in w:
-            continue
-        t = f[k]["FETCH_SIZE"] * 1024.0 * fs + w[k]["WRITE_SIZE"] * 1024.0 * ws
-        e = dict(traffic_bytes=round(t), traffic_bytes_per_particle=round(t / N, 1))
-        if q and k in q:
-            c = q[k]
-            cyc = c["GRBM_GUI_ACTIVE"] / 8.0  # per XCD: the kernel's cycles
-            e["valu_busy"] = round(c["SQ_INSTS_VALU"] * 4.0 / 1024.0 / cyc, 3)  # 1024 SIMDs, 4 cycles per wave64 op
-            e["ta_busy"] = round(c["TA_TA_BUSY"] / 256.0 / cyc, 3)
-            e["ta_cycles_per_load"] = round(c["TA_TA_BUSY"] / max(c["TA_FLAT_READ_WAVEFRONTS"], 1.0), 1)
-        out["kernels"][k] = e
-    return out
+    if d["fetch_scale"] is None:
+        return None
+    for k, e in d["kernels"].items():
+        if "traffic_bytes" in e:
+            e["traffic_bytes_per_particle"] = round(e["traffic_bytes"] / N, 1)
+    return d
 
 
 def cpu_baseline(sim, scene, iterations, threads):
@@ -432,6 +370,12 @@ def main():
     print(f"[bench] {N} particles, {sim.numCells} cells; settling {args.settle} substeps", file=sys.stderr,
           flush=True)
     sim.step(args.settle, I, False)
+    ckpt = None
+    if not args.no_counters and args.config == "column1m":
+        import tempfile
+        from xpbi import checkpoint
+        ckpt = os.path.join(tempfile.mkdtemp(prefix="xpbi_bench_"), "settled.npz")
+        checkpoint.save(sim, ckpt, args.settle)
     blocks, tune_ms = sim.autotune(I, False, substeps=2)
     print(f"[bench] autotuned launch: {blocks} blocks (0 = one per run), {tune_ms:.3f} ms/substep",
           file=sys.stderr, flush=True)
@@ -495,18 +439,21 @@ def main():
     sim.close()
 
     counters = None
-    if not args.no_counters:
-        print("[bench] counter passes (rocprofv3 --pmc) of the native driver", file=sys.stderr, flush=True)
-        counters = counter_passes(N)
+    if ckpt:
+        print("[bench] counter passes (rocprofv3 --pmc), one substep each from the settled column",
+              file=sys.stderr, flush=True)
+        counters = counter_passes(N, ckpt)
+        import shutil
+        shutil.rmtree(os.path.dirname(ckpt), ignore_errors=True)
         if counters:
             for r in (roofline, roofline_other):
                 c = counters["kernels"].get("k_" + r["kernel"])
                 if c:
                     r["traffic"] = c["traffic_bytes"]
-                    r["traffic_source"] = ("rocprofv3 FETCH_SIZE + WRITE_SIZE of this build's native driver on the "
-                                           "same column (20 substeps from rest, one counted substep), calibrated on "
+                    r["traffic_source"] = ("rocprofv3 FETCH_SIZE + WRITE_SIZE of one substep of the same settled "
+                                           "column (xpbi.run_scene from a checkpoint of this run), calibrated on "
                                            "a 256 MiB copy")
-                    for key in ("valu_busy", "ta_busy", "ta_cycles_per_load"):
+                    for key in ("valu_busy", "ta_busy", "ta_cycles_per_load", "lines_per_load"):
                         if key in c:
                             r[key] = c[key]
 

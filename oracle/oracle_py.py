@@ -47,7 +47,7 @@ def _mk_state_struct(ctype):
 
 class _Opts(C.Structure):
     _fields_ = [("tol", C.c_double), ("wsMode", C.c_int), ("wsParams", C.c_void_p), ("feat", C.c_void_p),
-                ("target", C.c_void_p), ("itersDone", C.c_int)]
+                ("target", C.c_void_p), ("itersDone", C.c_int), ("lam0ById", C.c_void_p)]
 
 
 S32 = _mk_state_struct(C.c_float)
@@ -155,18 +155,24 @@ class State:
 
 
 def substep(state, params, mats, iterations, has_water, stop="FULL", nthreads=None, resid=None, tol=0.0,
-            warm=None, collect=False, feat=None, target=None):
+            warm=None, collect=False, feat=None, target=None, lam0_by_id=None):
     """Run one (partial) substep in place.  params/mats: ctypes records (bytes-compatible).
 
     tol > 0: `iterations` is a cap; the Jacobi loop stops after the first
     iteration whose residual ||num|| / ||C at lambda = 0|| is <= tol.
     warm: canonical MLP parameters -> learned warm start; collect: training
-    pairs.  feat (N x 16) and target (N x 2) receive the feature rows and targets
+    pairs; lam0_by_id: a given lambda0 per persistent particle id, applied as the
+    first iteration's Delta-lambda where the constraint is active (the ceiling
+    experiments of the warm start).  feat (N x 16) and target (N x 2) receive the feature rows and targets
     (allocated when None).  Returns the number of Jacobi iterations applied."""
     st = state._struct()
     nt = nthreads or os.cpu_count() or 1
     rp = resid.ctypes.data if resid is not None else None
-    mode = 1 if warm is not None else (2 if collect else 0)
+    mode = 1 if warm is not None else (2 if collect else (3 if lam0_by_id is not None else 0))
+    lp = None
+    if lam0_by_id is not None:
+        lp = np.ascontiguousarray(lam0_by_id, state.dtype)
+        assert lp.shape == (state.N,)
     wp = None
     if warm is not None:
         wp = np.ascontiguousarray(warm, np.float32)
@@ -176,7 +182,8 @@ def substep(state, params, mats, iterations, has_water, stop="FULL", nthreads=No
     if mode == 2 and target is None:
         target = np.zeros((state.N, 2), state.dtype)
     o = _Opts(float(tol), mode, wp.ctypes.data if wp is not None else None,
-              feat.ctypes.data if feat is not None else None, target.ctypes.data if target is not None else None, 0)
+              feat.ctypes.data if feat is not None else None, target.ctypes.data if target is not None else None, 0,
+              lp.ctypes.data if lp is not None else None)
     f = lib().xo_substep_ex_f32 if state.dtype == np.float32 else lib().xo_substep_ex_f64
     rc = f(C.byref(st), C.addressof(params), C.addressof(mats), int(iterations), int(bool(has_water)),
            STAGE[stop] if isinstance(stop, str) else int(stop), int(nt), rp, C.byref(o))

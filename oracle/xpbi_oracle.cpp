@@ -90,6 +90,7 @@ struct xo_opts {
   void* feat;             // N x 16 feature rows (modes 1, 2), T
   void* target;           // N x 2 targets (mode 2), T
   int itersDone;          // out: Jacobi iterations applied
+  const void* lam0ById;   // mode 3: lambda0 per persistent particle id, T
 };
 
 template <class T, class ST>
@@ -109,12 +110,15 @@ static int run_substep(const ST* st, const void* params, const void* mats, int i
   sim.nthreads = nthreads > 0 ? nthreads : 1;
   sim.resid = resid;
   if (o) {
-    if ((o->wsMode == 1 && !o->wsParams) || (o->wsMode && !o->feat) || (o->wsMode == 2 && !o->target)) return -2;
+    if ((o->wsMode == 1 && !o->wsParams) || (o->wsMode && !o->feat) || (o->wsMode == 2 && !o->target) ||
+        (o->wsMode == 3 && !o->lam0ById))
+      return -2;
     sim.tol = o->tol;
     sim.wsMode = o->wsMode;
     sim.wsParams = o->wsParams;
     sim.feat = (T*)o->feat;
     sim.target = (T*)o->target;
+    sim.lam0ById = (const T*)o->lam0ById;
   }
   sim.substep(iters, hasWater != 0, stop);
   if (o) o->itersDone = sim.itersDone;

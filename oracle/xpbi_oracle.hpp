@@ -601,11 +601,13 @@ template <class T> struct Sim {
   // residual-driven loop (tol > 0: `iterations` is a cap; no reference counterpart)
   double tol = 0;
   int itersDone = 0;
-  // learned warm start (no reference counterpart): 0 plain, 1 learned, 2 collect pairs
+  // learned warm start (no reference counterpart): 0 plain, 1 learned, 2 collect
+  // pairs, 3 a given lambda0 per particle id (the warm-start ceiling experiments)
   int wsMode = 0;
   const float* wsParams = nullptr;  // canonical MLP parameters (mode 1)
-  T* feat = nullptr;                // N x 16 feature rows (modes 1, 2)
+  T* feat = nullptr;                // N x 16 feature rows (modes 1, 2, 3)
   T* target = nullptr;              // N x 2 targets (mode 2)
+  const T* lam0ById = nullptr;      // lambda0 indexed by persistent particle id (mode 3)
   std::vector<T> lamPrev, hbHint;
   T* velIn() { return flip ? S.velB : S.velA; }
   T* velOut() { return flip ? S.velA : S.velB; }
@@ -1111,6 +1113,16 @@ template <class T> struct Sim {
           lam0[i] = (row[12] == T(0) || y0 != y0) ? T(0) : wclamp(y0, T(0), T(2)) * row[12];
           hbHint[i] = T(float(y[1]));
         }
+        solveA(-1, lam0.data());
+        solveB();
+        if (hasWater) { pbfA(); pbfB(); }
+        warm = 1;
+      }
+      if (wsMode == 3 && iterations > 0) {
+        // the first Jacobi iteration applies Delta-lambda = the given lambda0 of
+        // each particle (zero where its constraint is inactive), then plain Jacobi
+        std::vector<T> lam0(P.N);
+        for (u32 i = 0; i < P.N; i++) lam0[i] = feat[16 * (size_t)i + 12] == T(0) ? T(0) : lam0ById[S.id[i]];
         solveA(-1, lam0.data());
         solveB();
         if (hasWater) { pbfA(); pbfB(); }

@@ -129,3 +129,38 @@ def test_collect_pairs(oracle):
     assert (feat[active, 14] >= 1e-7).all()
     # (about 33 support neighbours inside the column, fewer on its surface)
     assert (feat[active, 9] > 0).all() and 0.6 < np.median(feat[active, 9]) < 1.2
+
+
+def test_given_lambda0_equals_learned_start(oracle):
+    """The given-lambda0 start (lambda0 per particle id, the ceiling experiments of
+    oracle/warmstart_ceiling.py) fed the learned start's own lambda0 = clamp(y0, 0, 2) d0
+    of each particle reproduces the learned substep bit for bit; fed zero it applies no
+    multiplier in its first iteration."""
+    scene, P, mats, st = _landed(oracle)
+    net = init_params(5)
+    net[5120] = 0.7  # output bias: rho_hat near the plain first step, a non-trivial lambda0
+    a = st.copy()
+    feat = np.zeros((st.N, 16), np.float32)
+    na = oracle.substep(a, P, mats, 60, False, tol=1e-3, warm=net, feat=feat)
+    y = oracle.mlp_forward(net, feat[:, :12].astype(np.float64))
+    y0 = y[:, 0].astype(np.float32)
+    lam0 = np.where((feat[:, 12] == 0) | np.isnan(y0), np.float32(0),
+                    np.clip(y0, np.float32(0), np.float32(2)) * feat[:, 12]).astype(np.float32)
+    assert (lam0 != 0).mean() > 0.5
+    by_id = np.zeros(st.N, np.float32)
+    by_id[a.id] = lam0  # the substep's sorted order is the order a ends in
+    b = st.copy()
+    nb = oracle.substep(b, P, mats, 60, False, tol=1e-3, lam0_by_id=by_id)
+    assert na == nb
+    for f in ("pos", "velA", "F", "aux"):
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
+    # lambda0 = 0 with one iteration: the given start is the only iteration and
+    # leaves every multiplier at zero; a given lambda0 = 1 leaves it at 1 wherever
+    # the constraint is active
+    c = st.copy()
+    assert oracle.substep(c, P, mats, 1, False, lam0_by_id=np.zeros(st.N, np.float32)) == 1
+    assert not c.aux[:, 1].any()
+    c = st.copy()
+    f1 = np.zeros((st.N, 16), np.float32)
+    oracle.substep(c, P, mats, 1, False, lam0_by_id=np.ones(st.N, np.float32), feat=f1)
+    assert np.array_equal(c.aux[:, 1], (f1[:, 12] != 0).astype(np.float32))

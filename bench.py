@@ -82,6 +82,63 @@ def to_residual(sim, snap, params, tol, cap, substeps, before=None):
     return its, ms, out
 
 
+def warm_start_ceiling(sim, tol, cap, substeps):
+    """What any predictor of the first iteration's multipliers could gain, on the
+    same workload: `substeps` consecutive substeps to residual `tol` from the
+    current state, each run three times from the same state -- plain (lambda = 0),
+    exact (the first iteration applies lambda* of that substep's plain solve,
+    per particle id: a perfect predictor of lambda0) and previous (lambda of the
+    previous substep's solve, free at run time) -- through xpbi_set_lambda0.
+    Iterations count the given first iteration.  Returns mean iterations, ratios
+    and wall ms per substep (synchronised around each substep)."""
+    import numpy as np
+    keys = ("POS", "VEL", "F", "AUX", "ID")
+    sim.set_warmstart(None)
+    sim.set_tolerance(tol, 1)
+
+    def by_id(vals, ids):
+        out = np.zeros(sim.N, np.float32)
+        out[ids] = vals
+        return out
+
+    def one(state, lam0):
+        for k, v in state.items():
+            sim.write(k, v)
+        sim.set_lambda0(lam0)
+        sim.sync()
+        t0 = time.perf_counter()
+        sim.substep(cap, False)
+        sim.sync()
+        ms = (time.perf_counter() - t0) * 1e3
+        return sim.last_iterations()[0], ms
+    states, stars, prevs, plain = [], [], [], []
+    for _ in range(substeps):
+        st = {k: sim.read(k) for k in keys}
+        states.append(st)
+        prevs.append(by_id(st["AUX"][:, 1], st["ID"]))
+        n, ms = one(st, None)
+        plain.append((n, ms))
+        stars.append(by_id(sim.read("AUX")[:, 1], sim.read("ID")))
+    after = {k: sim.read(k) for k in keys}
+    exact = [one(states[k], stars[k]) for k in range(substeps)]
+    prev = [one(states[k], prevs[k]) for k in range(substeps)]
+    sim.set_lambda0(None)
+    for k, v in after.items():
+        sim.write(k, v)
+    sim.set_tolerance(0.0, 4)
+    mean = lambda rows, j: float(np.mean([r[j] for r in rows]))
+    out = dict(tol=tol, cap=cap, substeps=substeps, plain_iterations=mean(plain, 0),
+               exact_lambda0_iterations=mean(exact, 0), previous_lambda0_iterations=mean(prev, 0),
+               plain_ms=round(mean(plain, 1), 3), exact_lambda0_ms=round(mean(exact, 1), 3),
+               previous_lambda0_ms=round(mean(prev, 1), 3))
+    out["exact_iteration_ratio"] = round(out["plain_iterations"] / out["exact_lambda0_iterations"], 3)
+    out["previous_iteration_ratio"] = round(out["plain_iterations"] / out["previous_lambda0_iterations"], 3)
+    out["exact_time_ratio"] = round(out["plain_ms"] / out["exact_lambda0_ms"], 3)
+    out["note"] = ("exact: the first iteration applies each particle's converged lambda of the plain solve of the "
+                   "same substep (a perfect lambda0 predictor); previous: the last substep's converged lambda")
+    return out
+
+
 def trained_network(weights=None, log=None):
     """The warm-start network: trained here on the GPU (pairs of the training columns),
     or loaded from `weights`.  Returns (params, training record)."""
@@ -426,6 +483,8 @@ def main():
         resid = learned_vs_plain(sim, 1e-4, args.residual_cap, args.steps, net, train, warmup=args.warmup)
         ms_per_step = resid["learned_ms_per_substep"]
         value = N / (ms_per_step / 1e3)
+        print("[bench] warm-start ceiling: plain vs exact and previous lambda0", file=sys.stderr, flush=True)
+        resid["ceiling"] = warm_start_ceiling(sim, 1e-4, args.residual_cap, 8)
 
     copy_bytes = 1 << 30
     copy_ms = sim.copy_bandwidth(copy_bytes, reps=5)

@@ -318,6 +318,15 @@ int xpbi_set_warmstart(xpbi_sim* sim, const float* params, size_t n);
  * lambda = 0 (0 without an active constraint).  rows = N. */
 int xpbi_collect_warmstart_pairs(xpbi_sim* sim, int iterations, int has_water, float* x, float* t, float* w,
                                  size_t rows);
+/* Start every substep's Jacobi from given multipliers (no counterpart in the
+ * reference: the warm start's insertion point, simkernels.js:302-307, fed by the
+ * caller, e.g. with the multipliers an earlier solve converged to): from the
+ * next substep on, the first iteration applies Delta-lambda = lambda0[id] to
+ * each solid particle whose constraint is active (id: its persistent id,
+ * XPBI_FIELD_ID), then plain Jacobi follows.  n = N copies lambda0 to the
+ * device; lambda0 = NULL with n = 0 turns it off.  It replaces a network given
+ * to xpbi_set_warmstart, and a network given later replaces it. */
+int xpbi_set_lambda0(xpbi_sim* sim, const float* lambda0, size_t n);
 
 /* The MLP trainer: forward and backward on BF16 MFMA, Adam (beta 0.9, 0.999,
  * eps 1e-8) on f32 master weights.  Loss = mean over rows of

@@ -173,3 +173,39 @@ def test_trained_network_gains():
     assert ratio >= 1.15
     assert dpos <= 1e-4
     sim.close()
+
+
+@pytest.mark.parametrize("name", ["Sand: column collapse", "Dam breach: water + sand"])
+def test_given_lambda0_matches_oracle(oracle, name):
+    """xpbi_set_lambda0 against the oracle's given start on the same snapshot: the
+    first iteration applies lambda0 of each particle's id (here 1.3 times the
+    multipliers of the previous substep), then plain Jacobi."""
+    scene, sim = _settled(name)
+    I, water = scene["iterations"], scene["hasWater"]
+    st32 = snapshot(sim, oracle)
+    st64 = st32.copy(np.float64)
+    lam = np.zeros(sim.N, np.float32)
+    lam[sim.read("ID")] = sim.read("AUX")[:, 1]
+    assert (lam != 0).mean() > 0.2
+    lam0 = (1.3 * lam).astype(np.float32)
+    variants = [st32] + [perturbed(st32, k) for k in (1, 2)]
+    mats = mats_record(scene)
+    sim.set_lambda0(lam0)
+    sim.substep(I, water)
+    for st in variants + [st64]:
+        oracle.substep(st, sim.params, mats, I, water, lam0_by_id=lam0)
+    assert np.array_equal(sim.read("PERM"), st32.sortedIdx)
+    compare(sim, variants, st64, f"given lambda0 {name}")
+    # the given start set lambda of the first iteration: with one iteration only,
+    # lambda ends at lambda0 of the id wherever the constraint is active
+    snap = {k: sim.read(k) for k in ("POS", "VEL", "F", "AUX", "ID")}
+    sim.substep(1, water)
+    ids, lam1 = sim.read("ID"), sim.read("AUX")[:, 1]
+    act = lam1 != 0
+    assert act.mean() > 0.2 and np.array_equal(lam1[act], lam0[ids[act]])
+    sim.set_lambda0(None)
+    for k, v in snap.items():
+        sim.write(k, v)
+    sim.substep(1, water)
+    assert not np.array_equal(sim.read("AUX")[:, 1], lam1)
+    sim.close()

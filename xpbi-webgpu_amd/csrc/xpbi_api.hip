@@ -262,7 +262,7 @@ int xpbi_create_batch(const xpbi_scene_desc* descs, int count, xpbi_sim** out) {
   A(dalloc(s, &s->L.c, N));
   A(dalloc(s, &s->Sp.a, N));
   A(dalloc(s, &s->Sp.b, N));
-  A(dalloc(s, &s->Sp.c, N));
+  A(dalloc(s, &s->Sp.cx, N));
   A(dalloc(s, &s->selfDv, N));
   A(dalloc(s, &s->nbrLst, (size_t)s->nbrK * N));
   A(dalloc(s, &s->nbrHead, N));
@@ -276,6 +276,7 @@ int xpbi_create_batch(const xpbi_scene_desc* descs, int count, xpbi_sim** out) {
   A(dalloc(s, &s->radixHist, nr > 0 ? nr : 1));
   A(dalloc(s, &s->dMats, 8ull * count));
   A(dalloc(s, &s->dKP, (size_t)count));
+  A(dalloc(s, &s->vmaxSlot, 2ull * count));
   A(dalloc(s, &s->dTab, gN));
   A(dalloc(s, &s->residPart, 2ull * (gN > 0 ? gN : 1)));
   A(dalloc(s, &s->residOut, 2ull * kMaxResidIters));
@@ -489,13 +490,19 @@ int xpbi_read(xpbi_sim* h, int field, void* dst, size_t bytes) {
     case XPBI_FIELD_SP: {
       std::vector<float> a(4ull * N), b(4ull * N), c(N), w;
       const float4 *pa, *pb;
-      const float* pc;
+      const float* pc = nullptr;
       if (field == XPBI_FIELD_F) { pa = S.Fa; pb = S.Fb; pc = S.Fc; }
       else if (field == XPBI_FIELD_L) { pa = s->L.a; pb = s->L.b; pc = s->L.c; }
-      else { pa = s->Sp.a; pb = s->Sp.b; pc = s->Sp.c; }
+      else { pa = s->Sp.a; pb = s->Sp.b; }
       if ((r = d2h(s, a.data(), pa, 16ull * N))) return r;
       if ((r = d2h(s, b.data(), pb, 16ull * N))) return r;
-      if ((r = d2h(s, c.data(), pc, 4ull * N))) return r;
+      if (pc) {
+        if ((r = d2h(s, c.data(), pc, 4ull * N))) return r;
+      } else {  // Sp's ninth float is the w lane of (x_i, c)
+        std::vector<float> cx(4ull * N);
+        if ((r = d2h(s, cx.data(), s->Sp.cx, 16ull * N))) return r;
+        for (u32 i = 0; i < N; i++) c[i] = cx[4ull * i + 3];
+      }
       if (field == XPBI_FIELD_SP) {
         w.resize(4ull * N);
         if ((r = d2h(s, w.data(), s->selfDv, 16ull * N))) return r;
@@ -766,6 +773,27 @@ int xpbi_set_warmstart(xpbi_sim* h, const float* params, size_t n) {
   return XPBI_OK;
 }
 
+int xpbi_set_lambda0(xpbi_sim* h, const float* lambda0, size_t n) {
+  if (!h) return XPBI_ERR_ARG;
+  Sim* s = &h->s;
+  if (!lambda0) {
+    if (n != 0) return fail(s, XPBI_ERR_ARG, "lambda0 is null");
+    if (s->wsMode == 3) s->wsMode = 0;
+    return XPBI_OK;
+  }
+  if (n != (size_t)s->N) return fail(s, XPBI_ERR_ARG, "n must equal the particle count");
+  int r = ensure_warmstart_buffers(s);
+  if (r) return r;
+  if (!s->lam0ById && dalloc(s, &s->lam0ById, (size_t)s->N) != hipSuccess) {
+    s->lam0ById = nullptr;
+    return fail(s, XPBI_ERR_NOMEM, "lambda0 buffer: device allocation failed");
+  }
+  if ((r = xpbi_sync(h))) return r;
+  if ((r = h2d(s, s->lam0ById, lambda0, 4ull * n))) return r;
+  s->wsMode = 3;
+  return XPBI_OK;
+}
+
 int xpbi_collect_warmstart_pairs(xpbi_sim* h, int iterations, int has_water, float* x, float* t, float* w,
                                  size_t rows) {
   if (!h) return XPBI_ERR_ARG;
@@ -985,7 +1013,7 @@ constexpr size_t kNaPerParticle = 2 * (16 + 16 + 16 + 4 + 16 + 4)  // the two pa
                                   + 2 * 16                          // velocities
                                   + 2 * 4 + 2 * 4                   // sort keys and values
                                   + 16 + 16 + 4                     // L
-                                  + 16 + 16 + 4 + 16                // Sp, selfDv
+                                  + 16 + 16 + 16 + 16               // Sp, selfDv
                                   + 2 * kNaSlots + 8                // neighbour list and heads
                                   + 16;                             // predicted positions
 // per-particle and per-cell arrays at the maximum sizes, plus 1 MiB for the
@@ -1116,7 +1144,7 @@ int xpbi_substep_noalloc(const xpbi_params* p, const xpbi_material* mats, int n,
     s.vals[k] = (u32*)take(4ull * N);
   }
   s.L = M3P{(float4*)take(16ull * N), (float4*)take(16ull * N), (float*)take(4ull * N)};
-  s.Sp = M3P{(float4*)take(16ull * N), (float4*)take(16ull * N), (float*)take(4ull * N)};
+  s.Sp = SpP{(float4*)take(16ull * N), (float4*)take(16ull * N), (float4*)take(16ull * N)};
   s.selfDv = (float4*)take(16ull * N);
   s.nbrK = kNaSlots;
   s.nbrLst = (uint16_t*)take(2ull * kNaSlots * N);
@@ -1128,6 +1156,7 @@ int xpbi_substep_noalloc(const xpbi_params* p, const xpbi_material* mats, int n,
   s.radixHist = (u32*)take(4ull * std::max<size_t>(nr, 1));
   s.scanPart = (u32*)take(4ull * np);
   s.dKP = (KP*)take(sizeof(KP));
+  s.vmaxSlot = (u32*)take(2 * sizeof(u32));
   s.dMats = (DevMat*)take(8 * sizeof(DevMat));
   s.dTab = (uint2*)take(sizeof(uint2) * s.nb);
   if (off > kNaScratchBytes) return fail(nullptr, XPBI_ERR_ARG, "scratch layout exceeds the scratch array");

@@ -170,6 +170,24 @@ __device__ __forceinline__ void stm(M3P m, u32 i, const f33& v) {
   m.b[i] = make_float4(v.c[1].y, v.c[1].z, v.c[2].x, v.c[2].y);
   m.c[i] = v.c[2].z;
 }
+// Sp = dlam GL of solveA (and the unscaled GL of the warm-start features): a and
+// b as in M3P, cx = (x_i, the ninth float), so that the all-solid gather of
+// solveB reads a neighbour's position and its Sp as three 16-byte records
+struct SpP {
+  float4* a;
+  float4* b;
+  float4* cx;
+};
+struct SpC {
+  const float4* a;
+  const float4* b;
+  const float4* cx;
+};
+__device__ __forceinline__ void stsp(SpP m, u32 i, const f33& v, float4 pi) {
+  m.a[i] = make_float4(v.c[0].x, v.c[0].y, v.c[0].z, v.c[1].x);
+  m.b[i] = make_float4(v.c[1].y, v.c[1].z, v.c[2].x, v.c[2].y);
+  m.cx[i] = make_float4(pi.x, pi.y, pi.z, v.c[2].z);
+}
 
 // ---------------------------------------------------------------- SPH kernels
 // common.js:63-69

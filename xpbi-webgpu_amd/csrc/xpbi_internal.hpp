@@ -103,8 +103,15 @@ struct Sim {
   int cur = 0;
   float4* vel[2] = {nullptr, nullptr};
   int vcur = 0;  // index of the current velocity ("velA")
-  M3P L{}, Sp{};
+  M3P L{};
+  SpP Sp{};
   float4* selfDv = nullptr;
+  // per scene, max |v| over the velocities an iteration's solveB reads (as f32
+  // bits), two slots: solveA (or the warm-start features pass) raises slot
+  // vslot, solveB reads it and clears the other for the next iteration; computeL
+  // clears both
+  u32* vmaxSlot = nullptr;
+  int vslot = 0;
   u32* cellStart = nullptr;  // C + 1
   u32* count = nullptr;      // C
   u32* keys[2] = {nullptr, nullptr};
@@ -149,6 +156,7 @@ struct Sim {
   float* wsFeat = nullptr;   // N x 16 feature rows of the current substep
   float* wsHb = nullptr;     // N: HB bracket hint (mode 1)
   float* wsTarget = nullptr; // N x 2 training targets (mode 2)
+  float* lam0ById = nullptr; // N: given lambda0 per persistent id (mode 3)
   float* lamPrev = nullptr;  // N: previous substep's final lambda, in the current order
 
   // profiling

@@ -272,7 +272,7 @@ __global__ __launch_bounds__(256) void k_mlp_pack(const float* __restrict__ w, M
 // staged the weights once per block measured slower (85 against 60 us at 1M
 // particles), its chunks' loads no longer overlapping across blocks.
 __global__ __launch_bounds__(kMlpBlock) void k_mlp_apply(const MlpPacked* __restrict__ Wg, const float* __restrict__ X,
-                                                         u32 N, float4* __restrict__ aux, M3P Sp,
+                                                         u32 N, float4* __restrict__ aux, SpP Sp,
                                                          float4* __restrict__ selfDv, float* __restrict__ hbT) {
   __shared__ MlpPacked W;
   stage_weights(W, Wg);
@@ -289,7 +289,7 @@ __global__ __launch_bounds__(kMlpBlock) void k_mlp_apply(const MlpPacked* __rest
     float4 A = Sp.a[n], B = Sp.b[n];
     Sp.a[n] = make_float4(A.x * lam, A.y * lam, A.z * lam, A.w * lam);
     Sp.b[n] = make_float4(B.x * lam, B.y * lam, B.z * lam, B.w * lam);
-    Sp.c[n] = Sp.c[n] * lam;
+    Sp.cx[n].w = Sp.cx[n].w * lam;
     const float4 sd = selfDv[n];
     selfDv[n] = make_float4(sd.x * lam, sd.y * lam, sd.z * lam, 0.0f);
     if (hbT) hbT[n] = a.y[1];

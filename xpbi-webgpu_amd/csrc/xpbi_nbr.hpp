@@ -1,6 +1,7 @@
 // Neighbour iteration (cell rows, support-neighbour list, batched gathers) and
 // the per-particle finishing code of the constraint sweeps.
 #pragma once
+#include <type_traits>
 #include "xpbi_internal.hpp"
 
 // Multiply-adds are fused only inside one source expression in this file, never
@@ -179,9 +180,23 @@ struct RowCursor {
 // overflowed takes its neighbours from a RowCursor instead: the same neighbours
 // in the same lanes and order, through the same single copy of the loads and
 // of use().
+//
+// forSupportLate adds a second, dependent gather: once a word's records are in,
+// late(b, rec, |dxv|^2, valid) may load more of neighbour b (the contacts of
+// solveB load the neighbour's velocity only for the few pairs close enough to
+// touch), and use(b, rec, lateRec, dxv, |dxv|^2) consumes both.
+struct NoLate {};
+template <class T> __host__ __device__ const T& declRef() noexcept;  // unevaluated operands only
 template <u32 kL, class Load, class Use>
 __device__ __forceinline__ void forSupport(const KP& P, const u32* __restrict__ cs, const float4* __restrict__ pos,
-                                           const NLc& nl, u32 i, f3 xi, u32 q, Load&& load, Use&& use) {
+                                           const NLc& nl, u32 i, f3 xi, u32 q, Load&& load, Use&& use);
+template <u32 kL, class Load, class Late, class Use>
+__device__ __forceinline__ void forSupportLate(const KP& P, const u32* __restrict__ cs,
+                                               const float4* __restrict__ pos, const NLc& nl, u32 i, f3 xi, u32 q,
+                                               Load&& load, Late&& late, Use&& use) {
+  using Rec = decltype(load(0u));
+  using LateRec = decltype(late(0u, declRef<Rec>(), 0.0f, false));
+  constexpr bool kLate = !std::is_same<LateRec, NoLate>::value;
   static_assert(kL == 4u || kL == 8u, "four or eight lanes per particle");
   constexpr int kE = 16 / (int)kL;  // the lane's slots per group
   const uint2 hd = nl.head[i];
@@ -220,18 +235,37 @@ __device__ __forceinline__ void forSupport(const KP& P, const u32* __restrict__ 
       }
       if (!v[0]) break;
     }
-    decltype(load(0u)) d[kE];
+    Rec d[kE];
 #pragma unroll
     for (int j = 0; j < kE; j++) d[j] = load(b[j]);
+    if constexpr (kLate) {
+      // each record's dependent load just before its use: holding all of a
+      // word's late records at once costs a wave per SIMD of occupancy
 #pragma unroll
-    for (int j = 0; j < kE; j++) {
-      if (v[j]) {
+      for (int j = 0; j < kE; j++) {
         const f3 dd = xi - xyz(d[j].p);
-        use(b[j], d[j], dd, dist2(dd));
+        const float r2 = dist2(dd);
+        const LateRec e = late(b[j], d[j], r2, v[j]);
+        if (v[j]) use(b[j], d[j], e, dd, r2);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < kE; j++) {
+        if (v[j]) {
+          const f3 dd = xi - xyz(d[j].p);
+          use(b[j], d[j], NoLate{}, dd, dist2(dd));
+        }
       }
     }
     if (!v[kE - 1]) break;
   }
+}
+template <u32 kL, class Load, class Use>
+__device__ __forceinline__ void forSupport(const KP& P, const u32* __restrict__ cs, const float4* __restrict__ pos,
+                                           const NLc& nl, u32 i, f3 xi, u32 q, Load&& load, Use&& use) {
+  using Rec = decltype(load(0u));
+  forSupportLate<kL>(P, cs, pos, nl, i, xi, q, load, [](u32, const Rec&, float, bool) { return NoLate{}; },
+                     [&](u32 b, const Rec& d, NoLate, f3 dd, float r2) { use(b, d, dd, r2); });
 }
 
 // sum over the kL lanes (4 or 8) of a particle: ((l0 + l1) + (l2 + l3)) for a
@@ -274,6 +308,42 @@ __device__ __forceinline__ Acc runPasses(float (*lds)[256], u32 first, u32 pEnd,
   __syncthreads();  // the next run of a persistent block rewrites the slots
   return own;
 }
+
+// The velocity bound of solveB's contact test: each thread keeps the largest |v|
+// of the particles it finished and the scenes of its runs (the same for every
+// thread of the block); at the end of the kernel the block's maximum raises the
+// slot of each of those scenes (an upper bound for each) by an atomic max on
+// the f32 bits, which order like the values for non-negative floats.  The
+// atomic is skipped when the slot already reads higher: a stale read only costs
+// a redundant atomic.
+struct SpeedBound {
+  float s = 0.0f;
+  u32 lo = ~0u, hi = 0u;
+  __device__ __forceinline__ void add(u32 scene, f3 v) {
+    s = fmaxf(s, len(v));
+    lo = min(lo, scene);
+    hi = max(hi, scene);
+  }
+  __device__ __forceinline__ void note(u32 scene) {
+    lo = min(lo, scene);
+    hi = max(hi, scene);
+  }
+  // every thread of the block calls it
+  __device__ __forceinline__ void raise(u32* slot) {
+    if (!slot) return;
+    __shared__ float wmax[4];
+    float m = s;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63u) == 0u) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0u && lo <= hi) {
+      const u32 b = __float_as_uint(fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3])));
+      for (u32 sc = lo; sc <= hi; sc++)
+        if (b > *(volatile u32*)(slot + sc)) atomicMax(slot + sc, b);
+    }
+  }
+};
 
 __device__ __forceinline__ void putF3(float (*lds)[256], int row, u32 slot, f3 v) {
   lds[row][slot] = v.x;
@@ -535,7 +605,7 @@ __device__ __forceinline__ Constraint evalConstraint(const KP& P, const MatLds& 
 // moments.  Writes Sp, selfDv, lambda; accumulates the residual numerator^2 and
 // C^2 into *r2, *c2.
 __device__ __forceinline__ void solveAFinish(const KP& P, const MatLds& T, u32 i, float4 pi, f3 vi, bool solid,
-                                             const Moments& mo, M3C Fm, M3C Lm, float4* __restrict__ aux, M3P Sp,
+                                             const Moments& mo, M3C Fm, M3C Lm, float4* __restrict__ aux, SpP Sp,
                                              float4* __restrict__ selfDv, float* r2, float* c2,
                                              float hbHint = -1.0f) {
   bool wrote = false;
@@ -547,7 +617,7 @@ __device__ __forceinline__ void solveAFinish(const KP& P, const MatLds& T, u32 i
       const float num = -c.C - P.alphaT * a.y - g * P.dt * c.cdot;
       const float dlam = P.omega * num / ((1.0f + g) * c.denom + P.alphaT);
       aux[i].y = a.y + dlam;
-      stm(Sp, i, mscale(c.GL, dlam));
+      stsp(Sp, i, mscale(c.GL, dlam), pi);
       f3 sd = c.sgrad * dlam;
       selfDv[i] = make_float4(sd.x, sd.y, sd.z, 0.0f);
       wrote = true;
@@ -556,7 +626,7 @@ __device__ __forceinline__ void solveAFinish(const KP& P, const MatLds& T, u32 i
     }
   }
   if (!wrote) {
-    stm(Sp, i, mzero<float>());
+    stsp(Sp, i, mzero<float>(), pi);
     selfDv[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
 }
@@ -599,30 +669,58 @@ __device__ __forceinline__ f3 applyBoundary(const KP& P, f3 xcIn, f3 vIn, f3 xn,
   return vv;
 }
 
+// a neighbour's records of solveB's constraint gather: p = (x_b, the ninth float
+// of Sp_b), a and b = Sp_b's first eight floats, and with water the tag of b
 struct SpGather {
-  float4 p, v, a, b;
-  float c;
+  float4 p, a, b;
+  float tag;
+};
+// a contact candidate's velocity and tag, loaded only when the pair may touch
+struct SpLate {
+  float4 v;
+  float tag;
+  bool near;
 };
 
-// simkernels.js:338-355 for one neighbour: constraint gather dv += V_i S_b gradW(x_b - x_i)
-// and the 0.75h contact on predicted positions
+// simkernels.js:338-355 for one neighbour: the constraint gather
+// dv += V_i S_b gradW(x_b - x_i) and the 0.75 h contact on predicted positions,
+// whose test |x_i + dt v_i - x_b - dt v_b| < 0.75 h cannot pass unless
+// |x_i - x_b| < 0.75 h + dt (|v_i| + |v_b|): past the bound rc2 on |dxv|^2 (with
+// the largest |v_b| of the velocities read) the pair's velocity is not loaded
+// and the test is not evaluated.  The bound carries a margin above the
+// rounding of the f32 test, so the skipped pairs are exactly those the test
+// rejects.
 template <bool kAllSolid>
 __device__ __forceinline__ void solveBPair(const KP& P, const MatLds& T, float Vi, const float* wc_i, f3 xci, f3 dxv,
-                                           float r2, float4 pb4, float4 vb, const f33& Spb, f3* dv, f3* corr) {
-  if (kAllSolid || solidTag(T.solidMask, pb4.w)) {
+                                           float r2, const SpGather& g, const SpLate& e, f3* dv, f3* corr) {
+  if (kAllSolid || solidTag(T.solidMask, g.tag)) {
+    const f33 Spb{{f3{g.a.x, g.a.y, g.a.z}, f3{g.a.w, g.b.x, g.b.y}, f3{g.b.z, g.b.w, g.p.w}}};
     const f3 dxp = -dxv;  // x_b - x_i
     *dv = *dv + Vi * mul(Spb, gradWfast(P, dxp, r2));
-    const f3 xcb = xyz(pb4) + P.dt * xyz(vb);
-    const f3 dxc = xci - xcb;
-    const float d2 = dist2(dxc);
-    const float minD = P.minD;
-    if (d2 < minD * minD && d2 > 1e-18f) {
-      float idist = xrsq(d2);
-      float dist = d2 * idist;
-      float w = wc_i[matOfTag(pb4.w)];
-      *corr = *corr + dxc * (idist * ((minD - dist) * w));
+    if (e.near) {
+      const f3 xcb = xyz(g.p) + P.dt * xyz(e.v);
+      const f3 dxc = xci - xcb;
+      const float d2 = dist2(dxc);
+      const float minD = P.minD;
+      if (d2 < minD * minD && d2 > 1e-18f) {
+        float idist = xrsq(d2);
+        float dist = d2 * idist;
+        float w = wc_i[matOfTag(e.tag)];
+        *corr = *corr + dxc * (idist * ((minD - dist) * w));
+      }
     }
   }
+}
+
+// |dxv|^2 past which a pair of particle i cannot touch in solveB's contact test
+// (see solveBPair): vmax bounds |v_b| over the velocities read; the margin covers
+// the f32 rounding of x + dt v and of the test's difference (a few ulp of the
+// coordinates) and of the bound itself
+__device__ __forceinline__ float contactBound2(const KP& P, f3 xi, f3 vi, float vmax) {
+  const float reach = P.dt * (len(vi) + vmax);
+  const float xmax = fmaxf(fmaxf(fabsf(xi.x), fabsf(xi.y)), fabsf(xi.z)) + P.support + reach;
+  const float rc = (P.minD + reach) * (1.0f + 1e-5f) + 4e-6f * xmax;
+  return rc * rc;
 }
 
 // lane q's share of the constraint gather dv += V_i Sp_b gradW(x_b - x_i) and
@@ -643,19 +741,33 @@ struct SolveBAcc {
     return SolveBAcc{getF3(lds, 0, slot), getF3(lds, 3, slot)};
   }
 };
+// All solid: three 16-byte records per neighbour, (x_b, Sp_b's ninth float) and
+// Sp_b's first eight floats; with water also the tag of b.  A contact candidate
+// then loads its velocity (and, all solid, its tag).
 template <u32 kL, bool kAllSolid>
 __device__ __forceinline__ void solveBGatherQ(const KP& P, const MatLds& T, u32 i, u32 q, float4 pi, float4 vi4,
-                                              const float4* __restrict__ pos, const float4* __restrict__ vin, M3C Sp,
-                                              const u32* __restrict__ cs, const NLc& nl, SolveBAcc& a) {
+                                              float vmax, const float4* __restrict__ pos,
+                                              const float4* __restrict__ vin, SpC Sp, const u32* __restrict__ cs,
+                                              const NLc& nl, SolveBAcc& a) {
   const float Vi = vi4.w;
   const f3 xi = xyz(pi);
   const f3 xci = xi + P.dt * xyz(vi4);
   const float* wc_i = T.wc[matOfTag(pi.w)];
-  forSupport<kL>(P, cs, pos, nl, i, xi, q,
-      [&](u32 b) { return SpGather{pos[b], vin[b], Sp.a[b], Sp.b[b], Sp.c[b]}; },
-      [&](u32 b, const SpGather& g, f3 dxv, float r2) {
-        const f33 Spb{{f3{g.a.x, g.a.y, g.a.z}, f3{g.a.w, g.b.x, g.b.y}, f3{g.b.z, g.b.w, g.c}}};
-        solveBPair<kAllSolid>(P, T, Vi, wc_i, xci, dxv, r2, g.p, g.v, Spb, &a.dv, &a.corr);
+  const float rc2 = contactBound2(P, xi, xyz(vi4), vmax);
+  const float* __restrict__ tags = (const float*)pos + 3;
+  forSupportLate<kL>(P, cs, pos, nl, i, xi, q,
+      [&](u32 b) { return SpGather{Sp.cx[b], Sp.a[b], Sp.b[b], kAllSolid ? 0.0f : tags[4 * (size_t)b]}; },
+      [&](u32 b, const SpGather& g, float r2, bool valid) {
+        SpLate e{make_float4(0.0f, 0.0f, 0.0f, 0.0f), 0.0f, false};
+        if (valid && !(r2 >= rc2) && (kAllSolid || solidTag(T.solidMask, g.tag))) {
+          e.v = vin[b];
+          e.tag = kAllSolid ? tags[4 * (size_t)b] : g.tag;
+          e.near = true;
+        }
+        return e;
+      },
+      [&](u32 b, const SpGather& g, const SpLate& e, f3 dxv, float r2) {
+        solveBPair<kAllSolid>(P, T, Vi, wc_i, xci, dxv, r2, g, e, &a.dv, &a.corr);
       });
 }
 

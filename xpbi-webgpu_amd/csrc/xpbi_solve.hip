@@ -44,7 +44,11 @@ template <bool kPersist>
 __global__ __launch_bounds__(256) void k_computeL(const KP* __restrict__ kps, WorkTab W, const DevMat* __restrict__ mats,
                                                   const float4* __restrict__ pos, const float4* __restrict__ vin,
                                                   const u32* __restrict__ cs, M3P L, u16* __restrict__ nLst,
-                                                  uint2* __restrict__ nHead, u32 K, u32 NT) {
+                                                  uint2* __restrict__ nHead, u32 K, u32 NT,
+                                                  u32* __restrict__ vmaxSlot, u32 nSlots) {
+  // both velocity-bound slots of every scene start the solve cleared
+  if (blockIdx.x == 0)
+    for (u32 k = threadIdx.x; k < nSlots; k += 256u) vmaxSlot[k] = 0u;
   // each thread's first kStage list entries, written out slot by slot at the
   // end so that a store covers consecutive particles (entries found during the
   // walk land in different slots per lane)
@@ -165,27 +169,31 @@ __global__ __launch_bounds__(256) void k_computeL(const KP* __restrict__ kps, Wo
 template <bool kPersist>
 __global__ __launch_bounds__(256) void k_solveA(const KP* __restrict__ kps, WorkTab W, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
                                                 const float4* __restrict__ vin, M3C Fm, M3C Lm,
-                                                float4* __restrict__ aux, M3P Sp, float4* __restrict__ selfDv,
+                                                float4* __restrict__ aux, SpP Sp, float4* __restrict__ selfDv,
                                                 const u32* __restrict__ cs, NLc nl, double* __restrict__ part,
                                                 const SolveCtl* __restrict__ ctl, u32 k,
-                                                const float* __restrict__ hbHint) {
+                                                const float* __restrict__ hbHint, u32* __restrict__ vmaxOut) {
   if (skipIteration(ctl, k)) return;
   __shared__ MatLds T;
   u32 curScene = ~0u;
   float r2 = 0.0f, c2 = 0.0f;
+  SpeedBound sb;
   forWork<kPersist>(W, [&](u32 scene, u32 first) {
     sceneMats(T, mats, scene, &curScene);
     const KP& P = kps[scene];
     const Moments mo = runMoments<kLanesA, true>(P, T, first, cs, pos, vin, nl);
     const u32 i = first + threadIdx.x;
+    sb.note(scene);
     if (i >= P.pEnd) return;
     const float4 pi = pos[i];
     const f3 vi = xyz(vin[i]);
+    sb.add(scene, vi);
     const u32 mi = matOfTag(pi.w);
     const bool solid = (T.solidMask >> mi) & 1u;
     const float hint = (hbHint && T.m[mi].kind == MAT_HB) ? hbHint[i] : -1.0f;
     solveAFinish(P, T, i, pi, vi, solid, mo, Fm, Lm, aux, Sp, selfDv, &r2, &c2, hint);
   });
+  sb.raise(vmaxOut);
   if (part) blockReduce2((double)r2, (double)c2, part + 2 * blockIdx.x);
 }
 
@@ -259,24 +267,30 @@ __global__ __launch_bounds__(256) void k_parity_fix(const SolveCtl* __restrict__
 // simkernels.js:315-366
 template <bool kPersist>
 __global__ __launch_bounds__(256) void k_solveB(const KP* __restrict__ kps, WorkTab W, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
-                                                const float4* __restrict__ vin, M3C Sp,
+                                                const float4* __restrict__ vin, SpC Sp,
                                                 const float4* __restrict__ selfDv, const u32* __restrict__ cs,
                                                 NLc nl, float4* __restrict__ vout, const SolveCtl* ctl,
-                                                u32 k, float4* __restrict__ xcOut, ResidTest rt) {
+                                                u32 k, float4* __restrict__ xcOut, ResidTest rt,
+                                                const u32* __restrict__ vmaxIn, u32* __restrict__ vmaxNext,
+                                                u32 nScenes) {
   if (skipIteration(ctl, k)) return;
   if (rt.part && blockIdx.x == 0) residTest(rt, k);
+  // the other slot, free since the previous solveB read it, for the next solveA
+  if (blockIdx.x == 0)
+    for (u32 s = threadIdx.x; s < nScenes; s += 256u) vmaxNext[s] = 0u;
   __shared__ MatLds T;
   u32 curScene = ~0u;
   forWork<kPersist>(W, [&](u32 scene, u32 first) {
     sceneMats(T, mats, scene, &curScene);
     const KP& P = kps[scene];
+    const float vmax = __uint_as_float(vmaxIn[scene]);
     __shared__ float lds[SolveBAcc::kF][256];
     const SolveBAcc a = runPasses<kLanesB, SolveBAcc>(lds, first, P.pEnd, [&](u32 i, u32 q, SolveBAcc& acc) {
       const float4 pi = pos[i];
       if (!((T.solidMask >> matOfTag(pi.w)) & 1u)) return;
       const float4 vi4 = vin[i];
-      if (allSolid(T)) solveBGatherQ<kLanesB, true>(P, T, i, q, pi, vi4, pos, vin, Sp, cs, nl, acc);
-      else solveBGatherQ<kLanesB, false>(P, T, i, q, pi, vi4, pos, vin, Sp, cs, nl, acc);
+      if (allSolid(T)) solveBGatherQ<kLanesB, true>(P, T, i, q, pi, vi4, vmax, pos, vin, Sp, cs, nl, acc);
+      else solveBGatherQ<kLanesB, false>(P, T, i, q, pi, vi4, vmax, pos, vin, Sp, cs, nl, acc);
     });
     const u32 i = first + threadIdx.x;
     if (i >= P.pEnd) return;
@@ -517,19 +531,22 @@ __global__ __launch_bounds__(256) void k_integrate(const KP* __restrict__ kps, W
 template <bool kPersist>
 __global__ __launch_bounds__(256) void k_features(const KP* __restrict__ kps, WorkTab W, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
                                                   const float4* __restrict__ vin, M3C Fm, M3C Lm,
-                                                  const float4* __restrict__ aux, M3P Sp, float4* __restrict__ selfDv,
+                                                  const float4* __restrict__ aux, SpP Sp, float4* __restrict__ selfDv,
                                                   const u32* __restrict__ cs, NLc nl,
                                                   const float* __restrict__ lamPrev, float* __restrict__ feat,
-                                                  double* __restrict__ part) {
+                                                  double* __restrict__ part, u32* __restrict__ vmaxOut) {
   __shared__ MatLds T;
   u32 curScene = ~0u;
   float c2 = 0.0f;
+  SpeedBound sb;
   forWork<kPersist>(W, [&](u32 scene, u32 first) {
     sceneMats(T, mats, scene, &curScene);
     const KP& P = kps[scene];
     const Moments mo = runMoments<kLanesA, true>(P, T, first, cs, pos, vin, nl);
     const u32 i = first + threadIdx.x;
+    sb.note(scene);
     if (i >= P.pEnd) return;
+    sb.add(scene, xyz(vin[i]));
     float row[16];
 #pragma unroll
     for (int k = 0; k < 16; k++) row[k] = 0.0f;
@@ -573,14 +590,14 @@ __global__ __launch_bounds__(256) void k_features(const KP* __restrict__ kps, Wo
         row[13] = c.hbT;
         row[14] = c.C;
         row[15] = -c.C - g * P.dt * c.cdot;
-        stm(Sp, i, c.GL);
+        stsp(Sp, i, c.GL, pi);
         selfDv[i] = make_float4(c.sgrad.x, c.sgrad.y, c.sgrad.z, 0.0f);
         c2 += c.C * c.C;  // a persistent grid's thread walks several runs
         wrote = true;
       }
     }
     if (!wrote) {
-      stm(Sp, i, mzero<float>());
+      stsp(Sp, i, mzero<float>(), pi);
       selfDv[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     float4* o = (float4*)(feat + (u64)i * kFeatStride);
@@ -589,6 +606,7 @@ __global__ __launch_bounds__(256) void k_features(const KP* __restrict__ kps, Wo
     o[2] = make_float4(row[8], row[9], row[10], row[11]);
     o[3] = make_float4(row[12], row[13], row[14], row[15]);
   });
+  sb.raise(vmaxOut);
   blockReduce2(0.0, (double)c2, part + 2 * blockIdx.x);
 }
 
@@ -608,6 +626,25 @@ __global__ __launch_bounds__(256) void k_resid_c0(const double* __restrict__ par
   }
 }
 
+// the given start (xpbi_set_lambda0): Delta-lambda = lambda0 of the particle's
+// id where its constraint is active, scaling the unscaled Sp and selfDv of the
+// features pass as the MLP epilogue does
+__global__ __launch_bounds__(256) void k_lam0_apply(const float* __restrict__ feat, const u32* __restrict__ id,
+                                                    const float* __restrict__ lam0, u32 N, float4* __restrict__ aux,
+                                                    SpP Sp, float4* __restrict__ selfDv) {
+  const u32 n = blockIdx.x * 256u + threadIdx.x;
+  if (n >= N) return;
+  const float d0 = feat[(u64)n * kFeatStride + 12];
+  const float lam = d0 == 0.0f ? 0.0f : lam0[id[n]];
+  aux[n].y = lam;
+  const float4 A = Sp.a[n], B = Sp.b[n];
+  Sp.a[n] = make_float4(A.x * lam, A.y * lam, A.z * lam, A.w * lam);
+  Sp.b[n] = make_float4(B.x * lam, B.y * lam, B.z * lam, B.w * lam);
+  Sp.cx[n].w = Sp.cx[n].w * lam;
+  const float4 sd = selfDv[n];
+  selfDv[n] = make_float4(sd.x * lam, sd.y * lam, sd.z * lam, 0.0f);
+}
+
 // training targets after the plain solve: lambda / d0 and the HB root position
 __global__ __launch_bounds__(256) void k_collect_targets(const float* __restrict__ feat, const float4* __restrict__ aux,
                                                          u32 N, float* __restrict__ tgt) {
@@ -621,6 +658,7 @@ __global__ __launch_bounds__(256) void k_collect_targets(const float* __restrict
 // ---------------------------------------------------------------- schedule
 static inline u32 cdiv(u64 a, u64 b) { return (u32)((a + b - 1) / b); }
 static inline M3C cm(const M3P& m) { return M3C{m.a, m.b, m.c}; }
+static inline SpC spc(const SpP& m) { return SpC{m.a, m.b, m.cx}; }
 // iterations the host enqueues past the last residual test it has seen
 constexpr int kPollLead = 2;
 
@@ -662,10 +700,16 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
   M3P F{S.Fa, S.Fb, S.Fc};
   hipStream_t st = s->stream;
   int vi = s->vcur;
+  const u32 nScenes = (u32)s->sc.size();
   XPBI_LAUNCH_W(k_computeL, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
                      (const float4*)s->vel[vi],
-                     (const u32*)s->cellStart, s->L, s->nbrLst, s->nbrHead, s->nbrK, s->N);
+                     (const u32*)s->cellStart, s->L, s->nbrLst, s->nbrHead, s->nbrK, s->N, s->vmaxSlot,
+                     2u * nScenes);
   const NLc nl{s->nbrHead, s->nbrLst, s->nbrK, s->N};
+  // the velocity-bound slot the next solveA raises and its solveB reads
+  s->vslot = 0;
+  auto slotW = [&]() { return s->vmaxSlot + (size_t)s->vslot * nScenes; };
+  auto slotNext = [&]() { return s->vmaxSlot + (size_t)(s->vslot ^ 1) * nScenes; };
   s->prof_mark("computeL");
   if (stop == XPBI_STAGE_COMPUTE_L) return;
   s->residIters = 0;
@@ -679,18 +723,25 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
   if (s->wsMode != 0) {
     XPBI_LAUNCH_W(k_features, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
                        (const float4*)s->vel[vi], cm(F), cm(s->L), (const float4*)S.aux, s->Sp, s->selfDv,
-                       (const u32*)s->cellStart, nl, (const float*)s->lamPrev, s->wsFeat, s->residPart);
+                       (const u32*)s->cellStart, nl, (const float*)s->lamPrev, s->wsFeat, s->residPart, slotW());
     s->prof_mark("features");
     if (loop) hipLaunchKernelGGL(k_resid_c0, dim3(1), dim3(256), 0, st, (const double*)s->residPart, gN, ctl);
-    if (s->wsMode == 1 && iterations > 0) {
-      // the first Jacobi iteration with Delta-lambda = lambda0 from the MLP
-      // instead of omega d0: its Sp, selfDv and lambda, then solveB (and PBF)
-      launch_mlp_apply(s);
+    if ((s->wsMode == 1 || s->wsMode == 3) && iterations > 0) {
+      // the first Jacobi iteration with Delta-lambda = lambda0 from the MLP (or
+      // given per particle) instead of omega d0: its Sp, selfDv and lambda, then
+      // solveB (and PBF)
+      if (s->wsMode == 1) {
+        launch_mlp_apply(s);
+      } else {
+        hipLaunchKernelGGL(k_lam0_apply, dim3(cdiv(s->N, 256)), dim3(256), 0, st, (const float*)s->wsFeat,
+                           (const u32*)S.id, (const float*)s->lam0ById, s->N, S.aux, s->Sp, s->selfDv);
+      }
       s->prof_mark("mlp");
       XPBI_LAUNCH_W(k_solveB, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
-                         (const float4*)s->vel[vi], cm(s->Sp), (const float4*)s->selfDv, (const u32*)s->cellStart,
+                         (const float4*)s->vel[vi], spc(s->Sp), (const float4*)s->selfDv, (const u32*)s->cellStart,
                          nl, s->vel[vi ^ 1], (const SolveCtl*)nullptr, 0u, water ? s->xc : (float4*)nullptr,
-                         ResidTest{});
+                         ResidTest{}, (const u32*)slotW(), slotNext(), nScenes);
+      s->vslot ^= 1;
       s->prof_mark("solveB");
       vi ^= 1;
       if (water) {
@@ -720,13 +771,15 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
     double* part = (out || loop) ? s->residPart : nullptr;
     XPBI_LAUNCH_W(k_solveA, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
                        (const float4*)s->vel[vi], cm(F), cm(s->L), S.aux, s->Sp, s->selfDv,
-                       (const u32*)s->cellStart, nl, part, (const SolveCtl*)ctl, (u32)k, hbHint);
+                       (const u32*)s->cellStart, nl, part, (const SolveCtl*)ctl, (u32)k, hbHint, slotW());
     s->prof_mark("solveA");
     if (out) s->residIters = k + 1;
     const ResidTest rt{(const double*)part, gN, out, ctl, tol2, loop ? s->mirror : (CtlMirror*)nullptr, gen};
     XPBI_LAUNCH_W(k_solveB, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
-                       (const float4*)s->vel[vi], cm(s->Sp), (const float4*)s->selfDv, (const u32*)s->cellStart,
-                       nl, s->vel[vi ^ 1], (const SolveCtl*)ctl, (u32)k, water ? s->xc : (float4*)nullptr, rt);
+                       (const float4*)s->vel[vi], spc(s->Sp), (const float4*)s->selfDv, (const u32*)s->cellStart,
+                       nl, s->vel[vi ^ 1], (const SolveCtl*)ctl, (u32)k, water ? s->xc : (float4*)nullptr, rt,
+                       (const u32*)slotW(), slotNext(), nScenes);
+    s->vslot ^= 1;
     s->prof_mark("solveB");
     vi ^= 1;
     if (water) {

@@ -487,6 +487,28 @@ napi_value SetWarmstart(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// setLambda0(handle, Float32Array(N) | null): the given start per particle id
+napi_value SetLambda0(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value a[2];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, a, nullptr, nullptr));
+  xpbi_sim* s = get_sim(env, a[0]);
+  if (!s) return nullptr;
+  void* p = nullptr;
+  size_t n = 0;
+  if (argc > 1) {
+    napi_valuetype t;
+    napi_typeof(env, a[1], &t);
+    if (t != napi_null && t != napi_undefined && !get_bytes(env, a[1], &p, &n)) {
+      napi_throw_type_error(env, nullptr, "setLambda0(handle, Float32Array | null)");
+      return nullptr;
+    }
+  }
+  int rc = xpbi_set_lambda0(s, (const float*)p, n / 4);
+  if (rc) return throw_xpbi(env, s, rc, "xpbi_set_lambda0");
+  return nullptr;
+}
+
 // graphStep(handle, substeps, iterations, hasWater)
 napi_value GraphStep(napi_env env, napi_callback_info info) {
   size_t argc = 4;
@@ -571,7 +593,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"timeSubsteps", TimeSubsteps}, {"sync", Sync},
       {"createBatch", CreateBatch}, {"sceneInfo", SceneInfo}, {"writeSceneMaterials", WriteSceneMaterials},
       {"writeSceneParams", WriteSceneParams}, {"setTolerance", SetTolerance}, {"lastIterations", LastIterations},
-      {"setWarmstart", SetWarmstart}, {"graphStep", GraphStep}, {"setLaunch", SetLaunch},
+      {"setWarmstart", SetWarmstart}, {"setLambda0", SetLambda0}, {"graphStep", GraphStep}, {"setLaunch", SetLaunch},
       {"getLaunch", GetLaunch}, {"autotune", Autotune}, {"setNeighbourSlots", SetNeighbourSlots},
   };
   for (auto& f : fns) {

@@ -110,6 +110,7 @@ SIGNATURES = {
     "xpbi_autotune": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
     "xpbi_last_iterations": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "xpbi_set_warmstart": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t]),
+    "xpbi_set_lambda0": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t]),
     "xpbi_collect_warmstart_pairs": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                                C.POINTER(C.c_float), C.c_size_t]),
     "xpbi_substep_noalloc": (C.c_int, [C.POINTER(Params), C.POINTER(Material), C.c_int, _P, _P, _P, _P, C.c_int,

@@ -295,6 +295,15 @@ class Simulation:
         p = np.ascontiguousarray(params, dtype=np.float32).ravel()
         check(lib().xpbi_set_warmstart(self._h, _fptr(p), p.size), self._h, "set_warmstart")
 
+    def set_lambda0(self, lam0_by_id):
+        """Start each substep's Jacobi from lambda0[id] (N floats indexed by persistent
+        particle id), or turn that off with None (xpbi_set_lambda0)."""
+        if lam0_by_id is None:
+            check(lib().xpbi_set_lambda0(self._h, None, 0), self._h, "set_lambda0")
+            return
+        a = np.ascontiguousarray(lam0_by_id, dtype=np.float32).ravel()
+        check(lib().xpbi_set_lambda0(self._h, _fptr(a), a.size), self._h, "set_lambda0")
+
     def collect_warmstart_pairs(self, iterations, hasWater):
         """One plain substep; returns (features N x 12, targets N x 2 with NaN = no target,
         residual weights N)."""

@@ -82,15 +82,17 @@ def to_residual(sim, snap, params, tol, cap, substeps, before=None):
     return its, ms, out
 
 
-def warm_start_ceiling(sim, tol, cap, substeps):
+def warm_start_ceiling(sim, start, tol, cap, substeps, before=None):
     """What any predictor of the first iteration's multipliers could gain, on the
-    same workload: `substeps` consecutive substeps to residual `tol` from the
-    current state, each run three times from the same state -- plain (lambda = 0),
+    same substeps: `substeps` consecutive substeps to residual `tol` from the
+    state `start` (the learned-vs-plain snapshot), each run three times from the
+    same state -- plain (lambda = 0),
     exact (the first iteration applies lambda* of that substep's plain solve,
     per particle id: a perfect predictor of lambda0) and previous (lambda of the
     previous substep's solve, free at run time) -- through xpbi_set_lambda0.
-    Iterations count the given first iteration.  Returns mean iterations, ratios
-    and wall ms per substep (synchronised around each substep)."""
+    Iterations count the given first iteration; before(k) places a moving
+    collider for substep k.  Returns mean iterations, ratios and wall ms per
+    substep (synchronised around each substep)."""
     import numpy as np
     keys = ("POS", "VEL", "F", "AUX", "ID")
     sim.set_warmstart(None)
@@ -101,9 +103,11 @@ def warm_start_ceiling(sim, tol, cap, substeps):
         out[ids] = vals
         return out
 
-    def one(state, lam0):
-        for k, v in state.items():
-            sim.write(k, v)
+    def one(state, lam0, k):
+        for key, v in state.items():
+            sim.write(key, v)
+        if before:
+            before(k)
         sim.set_lambda0(lam0)
         sim.sync()
         t0 = time.perf_counter()
@@ -112,16 +116,18 @@ def warm_start_ceiling(sim, tol, cap, substeps):
         ms = (time.perf_counter() - t0) * 1e3
         return sim.last_iterations()[0], ms
     states, stars, prevs, plain = [], [], [], []
-    for _ in range(substeps):
+    for k, v in start.items():
+        sim.write(k, v)
+    for j in range(substeps):
         st = {k: sim.read(k) for k in keys}
         states.append(st)
         prevs.append(by_id(st["AUX"][:, 1], st["ID"]))
-        n, ms = one(st, None)
+        n, ms = one(st, None, j)
         plain.append((n, ms))
         stars.append(by_id(sim.read("AUX")[:, 1], sim.read("ID")))
     after = {k: sim.read(k) for k in keys}
-    exact = [one(states[k], stars[k]) for k in range(substeps)]
-    prev = [one(states[k], prevs[k]) for k in range(substeps)]
+    exact = [one(states[k], stars[k], k) for k in range(substeps)]
+    prev = [one(states[k], prevs[k], k) for k in range(substeps)]
     sim.set_lambda0(None)
     for k, v in after.items():
         sim.write(k, v)
@@ -130,7 +136,8 @@ def warm_start_ceiling(sim, tol, cap, substeps):
     out = dict(tol=tol, cap=cap, substeps=substeps, plain_iterations=mean(plain, 0),
                exact_lambda0_iterations=mean(exact, 0), previous_lambda0_iterations=mean(prev, 0),
                plain_ms=round(mean(plain, 1), 3), exact_lambda0_ms=round(mean(exact, 1), 3),
-               previous_lambda0_ms=round(mean(prev, 1), 3))
+               previous_lambda0_ms=round(mean(prev, 1), 3), per_substep=dict(
+                   plain=[r[0] for r in plain], exact=[r[0] for r in exact], previous=[r[0] for r in prev]))
     out["exact_iteration_ratio"] = round(out["plain_iterations"] / out["exact_lambda0_iterations"], 3)
     out["previous_iteration_ratio"] = round(out["plain_iterations"] / out["previous_lambda0_iterations"], 3)
     out["exact_time_ratio"] = round(out["plain_ms"] / out["exact_lambda0_ms"], 3)
@@ -163,13 +170,14 @@ def learned_vs_plain(sim, tol, cap, substeps, params, train, before=None, warmup
     pit, pms, ppos = to_residual(sim, snap, None, tol, cap, substeps, before)
     lit, lms, lpos = to_residual(sim, snap, params, tol, cap, substeps, before)
     pi, li = float(np.mean(pit)), float(np.mean(lit))
+    ceiling = warm_start_ceiling(sim, snap, tol, cap, substeps, before)
     return dict(tol=tol, cap=cap, substeps=substeps, plain_iterations=pi, learned_iterations=li,
                 iteration_ratio=round(pi / li, 3), plain_ms_per_substep=round(pms, 4),
                 learned_ms_per_substep=round(lms, 4), time_ratio=round(pms / lms, 3),
                 plain_particle_substeps_per_s=round(sim.N / (pms / 1e3), 1),
                 learned_particle_substeps_per_s=round(sim.N / (lms / 1e3), 1),
                 final_position_max_diff_m=float(np.abs(ppos[:, :3] - lpos[:, :3]).max()),
-                note="learned iterations count the learned first iteration", training=train)
+                note="learned iterations count the learned first iteration", training=train, ceiling=ceiling)
 
 
 def host_cpu():
@@ -483,8 +491,6 @@ def main():
         resid = learned_vs_plain(sim, 1e-4, args.residual_cap, args.steps, net, train, warmup=args.warmup)
         ms_per_step = resid["learned_ms_per_substep"]
         value = N / (ms_per_step / 1e3)
-        print("[bench] warm-start ceiling: plain vs exact and previous lambda0", file=sys.stderr, flush=True)
-        resid["ceiling"] = warm_start_ceiling(sim, 1e-4, args.residual_cap, 8)
 
     copy_bytes = 1 << 30
     copy_ms = sim.copy_bandwidth(copy_bytes, reps=5)

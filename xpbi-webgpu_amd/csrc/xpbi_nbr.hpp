@@ -671,11 +671,18 @@ __device__ __forceinline__ f3 applyBoundary(const KP& P, f3 xcIn, f3 vIn, f3 xn,
 
 // a neighbour's records of solveB's constraint gather: p = (x_b, the ninth float
 // of Sp_b), a and b = Sp_b's first eight floats, and with water the tag of b
+// a neighbour's records of solveB: all solid, p = (x_b, the ninth float of Sp_b)
+// and a, b = Sp_b's first eight floats; with water also the neighbour's
+// position/tag p4 and velocity up front
 struct SpGather {
   float4 p, a, b;
-  float tag;
 };
-// a contact candidate's velocity and tag, loaded only when the pair may touch
+struct SpGatherW {
+  float4 p, v, a, b;
+  float c;
+};
+// an all-solid contact candidate's velocity and tag, loaded only when the pair
+// may touch
 struct SpLate {
   float4 v;
   float tag;
@@ -683,31 +690,29 @@ struct SpLate {
 };
 
 // simkernels.js:338-355 for one neighbour: the constraint gather
-// dv += V_i S_b gradW(x_b - x_i) and the 0.75 h contact on predicted positions,
-// whose test |x_i + dt v_i - x_b - dt v_b| < 0.75 h cannot pass unless
-// |x_i - x_b| < 0.75 h + dt (|v_i| + |v_b|): past the bound rc2 on |dxv|^2 (with
-// the largest |v_b| of the velocities read) the pair's velocity is not loaded
-// and the test is not evaluated.  The bound carries a margin above the
-// rounding of the f32 test, so the skipped pairs are exactly those the test
-// rejects.
-template <bool kAllSolid>
-__device__ __forceinline__ void solveBPair(const KP& P, const MatLds& T, float Vi, const float* wc_i, f3 xci, f3 dxv,
-                                           float r2, const SpGather& g, const SpLate& e, f3* dv, f3* corr) {
-  if (kAllSolid || solidTag(T.solidMask, g.tag)) {
-    const f33 Spb{{f3{g.a.x, g.a.y, g.a.z}, f3{g.a.w, g.b.x, g.b.y}, f3{g.b.z, g.b.w, g.p.w}}};
-    const f3 dxp = -dxv;  // x_b - x_i
-    *dv = *dv + Vi * mul(Spb, gradWfast(P, dxp, r2));
-    if (e.near) {
-      const f3 xcb = xyz(g.p) + P.dt * xyz(e.v);
-      const f3 dxc = xci - xcb;
-      const float d2 = dist2(dxc);
-      const float minD = P.minD;
-      if (d2 < minD * minD && d2 > 1e-18f) {
-        float idist = xrsq(d2);
-        float dist = d2 * idist;
-        float w = wc_i[matOfTag(e.tag)];
-        *corr = *corr + dxc * (idist * ((minD - dist) * w));
-      }
+// dv += V_i S_b gradW(x_b - x_i) and, when `near`, the 0.75 h contact on
+// predicted positions (xb, vb: the neighbour's position and velocity, tagB its
+// material tag).  All solid, the contact test |x_i + dt v_i - x_b - dt v_b| <
+// 0.75 h cannot pass unless |x_i - x_b| < 0.75 h + dt (|v_i| + |v_b|): past the
+// bound rc2 on |dxv|^2 (with the largest |v_b| of the velocities read) the
+// pair's velocity is not loaded and the test is not evaluated.  The bound
+// carries a margin above the rounding of the f32 test, so the skipped pairs are
+// exactly those the test rejects.
+__device__ __forceinline__ void solveBPair(const KP& P, float Vi, const float* wc_i, f3 xci, f3 dxv, float r2,
+                                           const f33& Spb, bool near, f3 xb, float4 vb, float tagB, f3* dv,
+                                           f3* corr) {
+  const f3 dxp = -dxv;  // x_b - x_i
+  *dv = *dv + Vi * mul(Spb, gradWfast(P, dxp, r2));
+  if (near) {
+    const f3 xcb = xb + P.dt * xyz(vb);
+    const f3 dxc = xci - xcb;
+    const float d2 = dist2(dxc);
+    const float minD = P.minD;
+    if (d2 < minD * minD && d2 > 1e-18f) {
+      float idist = xrsq(d2);
+      float dist = d2 * idist;
+      float w = wc_i[matOfTag(tagB)];
+      *corr = *corr + dxc * (idist * ((minD - dist) * w));
     }
   }
 }
@@ -742,8 +747,9 @@ struct SolveBAcc {
   }
 };
 // All solid: three 16-byte records per neighbour, (x_b, Sp_b's ninth float) and
-// Sp_b's first eight floats; with water also the tag of b.  A contact candidate
-// then loads its velocity (and, all solid, its tag).
+// Sp_b's first eight floats; a contact candidate then loads its velocity and
+// tag.  With water (whose pairs the solid test skips, and whose clumps put many
+// pairs within contact reach): position/tag, velocity and Sp up front.
 template <u32 kL, bool kAllSolid>
 __device__ __forceinline__ void solveBGatherQ(const KP& P, const MatLds& T, u32 i, u32 q, float4 pi, float4 vi4,
                                               float vmax, const float4* __restrict__ pos,
@@ -753,22 +759,33 @@ __device__ __forceinline__ void solveBGatherQ(const KP& P, const MatLds& T, u32 
   const f3 xi = xyz(pi);
   const f3 xci = xi + P.dt * xyz(vi4);
   const float* wc_i = T.wc[matOfTag(pi.w)];
-  const float rc2 = contactBound2(P, xi, xyz(vi4), vmax);
-  const float* __restrict__ tags = (const float*)pos + 3;
-  forSupportLate<kL>(P, cs, pos, nl, i, xi, q,
-      [&](u32 b) { return SpGather{Sp.cx[b], Sp.a[b], Sp.b[b], kAllSolid ? 0.0f : tags[4 * (size_t)b]}; },
-      [&](u32 b, const SpGather& g, float r2, bool valid) {
-        SpLate e{make_float4(0.0f, 0.0f, 0.0f, 0.0f), 0.0f, false};
-        if (valid && !(r2 >= rc2) && (kAllSolid || solidTag(T.solidMask, g.tag))) {
-          e.v = vin[b];
-          e.tag = kAllSolid ? tags[4 * (size_t)b] : g.tag;
-          e.near = true;
-        }
-        return e;
-      },
-      [&](u32 b, const SpGather& g, const SpLate& e, f3 dxv, float r2) {
-        solveBPair<kAllSolid>(P, T, Vi, wc_i, xci, dxv, r2, g, e, &a.dv, &a.corr);
-      });
+  if constexpr (kAllSolid) {
+    const float rc2 = contactBound2(P, xi, xyz(vi4), vmax);
+    forSupportLate<kL>(P, cs, pos, nl, i, xi, q,
+        [&](u32 b) { return SpGather{Sp.cx[b], Sp.a[b], Sp.b[b]}; },
+        [&](u32 b, const SpGather& g, float r2, bool valid) {
+          SpLate e{make_float4(0.0f, 0.0f, 0.0f, 0.0f), 0.0f, false};
+          if (valid && !(r2 >= rc2)) {
+            e.v = vin[b];
+            e.tag = pos[b].w;
+            e.near = true;
+          }
+          return e;
+        },
+        [&](u32 b, const SpGather& g, const SpLate& e, f3 dxv, float r2) {
+          const f33 Spb{{f3{g.a.x, g.a.y, g.a.z}, f3{g.a.w, g.b.x, g.b.y}, f3{g.b.z, g.b.w, g.p.w}}};
+          solveBPair(P, Vi, wc_i, xci, dxv, r2, Spb, e.near, xyz(g.p), e.v, e.tag, &a.dv, &a.corr);
+        });
+  } else {
+    const float* __restrict__ c9 = (const float*)Sp.cx + 3;
+    forSupport<kL>(P, cs, pos, nl, i, xi, q,
+        [&](u32 b) { return SpGatherW{pos[b], vin[b], Sp.a[b], Sp.b[b], c9[4 * (size_t)b]}; },
+        [&](u32 b, const SpGatherW& g, f3 dxv, float r2) {
+          if (!solidTag(T.solidMask, g.p.w)) return;
+          const f33 Spb{{f3{g.a.x, g.a.y, g.a.z}, f3{g.a.w, g.b.x, g.b.y}, f3{g.b.z, g.b.w, g.c}}};
+          solveBPair(P, Vi, wc_i, xci, dxv, r2, Spb, true, xyz(g.p), g.v, g.p.w, &a.dv, &a.corr);
+        });
+  }
 }
 
 // simkernels.js:357-365: impulse cap, contact relaxation, boundaries; with xcOut

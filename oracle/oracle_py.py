@@ -274,3 +274,19 @@ def cell_index(params, p):
     out = np.zeros(p.shape[0], np.uint32)
     lib().xo_cell_index(C.addressof(params), p.shape[0], _p(p), _p(out))
     return out
+
+
+SAN_DIR = os.path.join(HERE, "_san")
+
+
+def build_sanitized():
+    """The oracle under AddressSanitizer and UndefinedBehaviorSanitizer: the driver
+    oracle/sanitize_main.cpp linked with xpbi_oracle.cpp into oracle/_san/oracle_san
+    (a finding aborts the run).  Returns the executable's path."""
+    os.makedirs(SAN_DIR, exist_ok=True)
+    exe = os.path.join(SAN_DIR, "oracle_san")
+    cmd = ["g++", "-O1", "-g", "-std=c++17", "-fopenmp", "-ffp-contract=off", "-fno-omit-frame-pointer",
+           "-fsanitize=address,undefined", "-fno-sanitize-recover=all", os.path.join(HERE, "xpbi_oracle.cpp"),
+           os.path.join(HERE, "sanitize_main.cpp"), "-o", exe]
+    subprocess.run(cmd, check=True)
+    return exe

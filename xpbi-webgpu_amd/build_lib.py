@@ -69,6 +69,30 @@ def build_bench():
     return BENCH
 
 
+UBSAN_LIB = os.path.join(PKG, "ubsan", "libxpbi.so")
+
+
+def build_lib_ubsan():
+    """libxpbi.so with UndefinedBehaviorSanitizer on its host code only (GPU code is
+    built as usual), the shared sanitizer runtime found by rpath; a finding aborts.
+    Load it in place of the library with XPBI_LIB=<this path> (the Python host)."""
+    import glob
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    rt = glob.glob("/opt/rocm/lib/llvm/lib/clang/*/lib/linux/libclang_rt.ubsan_standalone-x86_64.so")
+    if not rt:
+        raise RuntimeError("no shared UBSan runtime under /opt/rocm/lib/llvm")
+    os.makedirs(os.path.dirname(UBSAN_LIB), exist_ok=True)
+    tmp = UBSAN_LIB + ".tmp"
+    cmd = [hipcc] + HIPCC_FLAGS + ["-I" + os.path.join(REPO, "include"),
+                                   "-Xarch_host", "-fsanitize=undefined",
+                                   "-Xarch_host", "-fno-sanitize-recover=undefined", "-shared-libsan",
+                                   "-Wl,-rpath," + os.path.dirname(rt[0])] + \
+        [os.path.join(CSRC, s) for s in SOURCES] + ["-o", tmp]
+    _run(cmd)
+    os.replace(tmp, UBSAN_LIB)
+    return UBSAN_LIB
+
+
 def build():
     build_lib()
     build_addon()
@@ -76,5 +100,8 @@ def build():
 
 
 if __name__ == "__main__":
-    build()
-    print(LIB)
+    if "--ubsan" in sys.argv:
+        print(build_lib_ubsan())
+    else:
+        build()
+        print(LIB)

@@ -227,9 +227,10 @@ def test_substep_parity_headline_column(oracle):
 
 def test_substep_parity_vm_poke_1m(oracle):
     """The 1M von Mises block (100^3 at r = 0.005) at full size. The 0.05 m sphere
-    moves down at 1 m/s and its position is rewritten every substep. After 200
-    substeps it has reached the block's top and pushes on it (a few hundred particles
-    lie within 2h of its surface); one substep there must meet the bar, with the
+    moves down at 1 m/s and its position is rewritten every substep; it starts 4
+    spacings above the falling block and after 400 substeps presses into it (960
+    particles within 2h of its surface and 190 inside it before the substep, against
+    418 and none after 200 substeps). One substep there must meet the bar, with the
     sphere's push in the boundary pass."""
     scene = scenes.vm_poke_1m()
     sim = xpbi.Simulation(scene, device=0)
@@ -239,18 +240,21 @@ def test_substep_parity_vm_poke_1m(oracle):
         pos = [col["pos"][j] + col["vel"][j] * k * dt for j in range(3)]
         sim.writeParams(xpbi.params_from_scene(scene, collider=dict(pos=pos, R=col["R"], vel=col["vel"])))
 
-    print("von Mises poke 1M: 200 substeps with the sphere moving", flush=True)
-    for k in range(200):
+    steps = 400
+    print(f"von Mises poke 1M: {steps} substeps with the sphere moving", flush=True)
+    for k in range(steps):
         place(k)
         sim.substep(scene["iterations"], False)
-    place(200)
+    place(steps)
     x = sim.read("POS")[:, :3]
-    c = np.asarray(col["pos"]) + np.asarray(col["vel"]) * 200 * dt
+    c = np.asarray(col["pos"]) + np.asarray(col["vel"]) * steps * dt
     d = np.linalg.norm(x - c, axis=1)
     near = int((d < col["R"] + 2 * scene["h"]).sum())
     print(f"particles within 2h of the sphere before the checked substep: {near}", flush=True)
-    assert near > 100, "the sphere does not touch the block"
-    one_substep_parity(sim, scene, oracle, "1M von Mises poke after 200 substeps", water=False)
+    inside = int((d < col["R"]).sum())
+    print(f"particles inside the sphere before the checked substep: {inside}", flush=True)
+    assert near > 800 and inside > 100, "the sphere does not press into the block"
+    one_substep_parity(sim, scene, oracle, f"1M von Mises poke after {steps} substeps", water=False)
     sim.close()
 
 

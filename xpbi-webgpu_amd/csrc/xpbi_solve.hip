@@ -69,61 +69,115 @@ __global__ __launch_bounds__(256) void k_computeL(const KP* __restrict__ kps, Wo
     const bool wantWater = waterMask != 0u && ((T.solidMask >> matOfTag(pos[i].w)) & 1u);
     float tmin = 3.0e38f, tmax = -3.0e38f;
     f3 xi = xyz(pos[i]);
-    f33 A = mzero<float>();
     u32 n = 0, flag = 0;
-    // the 9 row spans of forRowSpans, entries relative to the first candidate
-    // (the lowest cell of the stencil); candidates in batches of 8: their
-    // positions, then the volumes of the accepted ones, each batch's loads
-    // issued together; accumulation and the list keep the candidate order
+    // The 9 row spans of forRowSpans, entries relative to the first candidate
+    // (the lowest cell of the stencil).  The candidate walk only tests distances
+    // and appends the accepted candidates to the list (candidates in batches of
+    // 8, each batch's loads issued together); A is then summed over the list in
+    // list order, which is the candidate order.  Summing inside the walk would
+    // run the accumulation for every candidate slot in which any lane of the
+    // wave accepts (nearly all of the ~216), instead of the ~30 neighbours.
     const i3 cc = cellCoord(P, xi);
     const u32 x0 = (u32)max(cc.x - 1, 0), x1 = (u32)min(cc.x + 1, P.gx - 1);
     const u32 y0 = (u32)max(cc.y - 1, 0), z0 = (u32)max(cc.z - 1, 0);
     const u32 base = cs[P.cellBase + z0 * P.gxy + y0 * (u32)P.gx + x0];
+    // in registers: the list stores below may alias the scene record for all the
+    // compiler knows, which would reload these from memory at every candidate
+    const float supp2 = P.supp2;
+    const int gy = P.gy, gz = P.gz;
+    const u32 gxy = P.gxy, gx = (u32)P.gx, cellBase = P.cellBase;
+    // kTrack: the scene has water, so a solid particle records the range of its
+    // candidates' tags
+    auto walk = [&](auto trackTag) {
+      constexpr bool kTrack = decltype(trackTag)::value;
 #pragma unroll 1
-    for (int dz = -1; dz <= 1; dz++) {
-      const int z = cc.z + dz;
-      if (z < 0 || z >= P.gz) continue;
-      const u32 plane = P.cellBase + (u32)z * P.gxy;
+      for (int dz = -1; dz <= 1; dz++) {
+        const int z = cc.z + dz;
+        if (z < 0 || z >= gz) continue;
+        const u32 plane = cellBase + (u32)z * gxy;
 #pragma unroll 1
-      for (int dy = -1; dy <= 1; dy++) {
-        const int y = cc.y + dy;
-        if (y < 0 || y >= P.gy) continue;
-        const u32 row = plane + (u32)y * (u32)P.gx;
-        const u32 beg = cs[row + x0];
-        const u32 end = min(cs[row + x1 + 1u], beg + kRowCap);
-        for (u32 b0 = beg; b0 < end; b0 += 8u) {
-          float4 pb[8];
+        for (int dy = -1; dy <= 1; dy++) {
+          const int y = cc.y + dy;
+          if (y < 0 || y >= gy) continue;
+          const u32 row = plane + (u32)y * gx;
+          const u32 beg = cs[row + x0];
+          const u32 end = min(cs[row + x1 + 1u], beg + kRowCap);
+#pragma unroll 1
+          for (u32 b0 = beg; b0 < end; b0 += 8u) {
+            // past the span's end the loads repeat its last candidate (no
+            // predicated loads), which the test below rejects
+            float4 pb[8];
 #pragma unroll
-          for (int j = 0; j < 8; j++) pb[j] = (b0 + j < end) ? pos[b0 + j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-          f3 d[8];
-          float r2[8];
-          bool ok[8];
+            for (int j = 0; j < 8; j++) pb[j] = gat(pos, min(b0 + (u32)j, end - 1u));
 #pragma unroll
-          for (int j = 0; j < 8; j++) {
-            d[j] = xi - xyz(pb[j]);
-            r2[j] = dist2(d[j]);
-            ok[j] = (b0 + j < end) && (b0 + j != i) && (r2[j] < P.supp2);
-            if (wantWater && b0 + j < end) {
-              tmin = fminf(tmin, pb[j].w);
-              tmax = fmaxf(tmax, pb[j].w);
+            for (int j = 0; j < 8; j++) {
+              const u32 c = b0 + (u32)j;
+              const bool in = c < end;
+              if (kTrack && wantWater && in) {
+                tmin = fminf(tmin, pb[j].w);
+                tmax = fmaxf(tmax, pb[j].w);
+              }
+              if (in && c != i && dist2(xi - xyz(pb[j])) < supp2) {
+                const u32 off = c - base;
+                flag |= off > kNbrOffMax ? kNbrFlag : 0u;
+                const u16 e = (u16)off;
+                if (n < kStage) stage[n][tid] = e;
+                else if (n < K) nLst[slotAt(n, i, NT)] = e;
+                n++;
+              }
             }
           }
-          float Vb[8];
+        }
+      }
+    };
+    if (waterMask != 0u) walk(std::true_type{});
+    else walk(std::false_type{});
+    // A = sum_b V_b gradW(x_i - x_b) (x) (x_b - x_i) in candidate order
+    f33 A = mzero<float>();
+    auto addPair = [&](float4 pb, float Vb) {
+      const f3 d = xi - xyz(pb);
+      const f3 g = gradWfast(P, d, dist2(d));
+      A = madd(A, mscale(outer(g, -d), Vb));
+    };
+    if (!(flag & kNbrFlag) && n <= max(K, kStage)) {
+      // every neighbour is in the stage (slots below kStage) or in the list
+#pragma unroll 1
+      for (u32 k0 = 0; k0 < n; k0 += 8u) {
+        u32 b[8];
 #pragma unroll
-          for (int j = 0; j < 8; j++) Vb[j] = ok[j] ? vin[b0 + j].w : 0.0f;
+        for (int j = 0; j < 8; j++) {
+          const u32 k = k0 + (u32)j;
+          b[j] = i;
+          if (k < n) b[j] = base + (u32)(k < kStage ? stage[k][tid] : nLst[slotAt(k, i, NT)]);
+        }
+        float4 pb[8];
+        float Vb[8];
 #pragma unroll
-          for (int j = 0; j < 8; j++) {
-            if (ok[j]) {
-              f3 g = gradWfast(P, d[j], r2[j]);
-              f3 r = -d[j];
-              A = madd(A, mscale(outer(g, r), Vb[j]));
-              const u32 off = b0 + (u32)j - base;
-              flag |= off > kNbrOffMax ? kNbrFlag : 0u;
-              const u16 e = (u16)off;
-              if (n < kStage) stage[n][tid] = e;
-              else if (n < K) nLst[slotAt(n, i, NT)] = e;
-              n++;
-            }
+        for (int j = 0; j < 8; j++) {
+          pb[j] = pos[b[j]];
+          Vb[j] = vin[b[j]].w;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+          if (k0 + (u32)j < n) addPair(pb[j], Vb[j]);
+      }
+    } else {
+      // the list cannot hold them all: the candidates again, one at a time
+#pragma unroll 1
+      for (int dz = -1; dz <= 1; dz++) {
+        const int z = cc.z + dz;
+        if (z < 0 || z >= P.gz) continue;
+#pragma unroll 1
+        for (int dy = -1; dy <= 1; dy++) {
+          const int y = cc.y + dy;
+          if (y < 0 || y >= P.gy) continue;
+          const u32 row = P.cellBase + (u32)z * P.gxy + (u32)y * (u32)P.gx;
+          const u32 beg = cs[row + x0];
+          const u32 end = min(cs[row + x1 + 1u], beg + kRowCap);
+#pragma unroll 1
+          for (u32 c = beg; c < end; c++) {
+            const float4 pb = pos[c];
+            if (c != i && dist2(xi - xyz(pb)) < P.supp2) addPair(pb, vin[c].w);
           }
         }
       }
@@ -154,12 +208,30 @@ __global__ __launch_bounds__(256) void k_computeL(const KP* __restrict__ kps, Wo
       dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
     }
     nHead[i] = make_uint2(base, n | flag);
-    SVD3<double> s = svd3<double>(mcast<double>(A));
-    d3 inv = splat3(0.0);
-    if (fabs(s.sig.x) > 0.2) inv.x = 1.0 / s.sig.x;
-    if (fabs(s.sig.y) > 0.2) inv.y = 1.0 / s.sig.y;
-    if (fabs(s.sig.z) > 0.2) inv.z = 1.0 / s.sig.z;
-    stm(L, i, mcast<float>(diagMul(s.V, inv, s.U)));
+    // L = pinv(A) with singular values below 0.2 dropped, in FP64.  When
+    // Gershgorin's bound on A^T A puts every singular value above the cut (with
+    // a margin far above FP64 rounding), pinv(A) = A^-1, formed from cofactors:
+    // row k of A^-1 is (c_{k+1} x c_{k+2}) / det for the columns c of A.  Else
+    // the SVD (simkernels.js:210-220).
+    const d33 Ad = mcast<double>(A);
+    const d3 a0 = Ad.c[0], a1 = Ad.c[1], a2 = Ad.c[2];
+    const double b00 = dot(a0, a0), b11 = dot(a1, a1), b22 = dot(a2, a2);
+    const double b01 = fabs(dot(a0, a1)), b02 = fabs(dot(a0, a2)), b12 = fabs(dot(a1, a2));
+    const double lmin = fmin(fmin(b00 - b01 - b02, b11 - b01 - b12), b22 - b02 - b12);
+    if (lmin > 0.04 * (1.0 + 1e-6)) {
+      const d3 r0 = cross(a1, a2), r1 = cross(a2, a0), r2 = cross(a0, a1);
+      const double id = 1.0 / dot(a0, r0);
+      const d33 Ai{{d3{r0.x * id, r1.x * id, r2.x * id}, d3{r0.y * id, r1.y * id, r2.y * id},
+                    d3{r0.z * id, r1.z * id, r2.z * id}}};
+      stm(L, i, mcast<float>(Ai));
+    } else {
+      SVD3<double> s = svd3<double>(Ad);
+      d3 inv = splat3(0.0);
+      if (fabs(s.sig.x) > 0.2) inv.x = 1.0 / s.sig.x;
+      if (fabs(s.sig.y) > 0.2) inv.y = 1.0 / s.sig.y;
+      if (fabs(s.sig.z) > 0.2) inv.z = 1.0 / s.sig.z;
+      stm(L, i, mcast<float>(diagMul(s.V, inv, s.U)));
+    }
   });
 }
 

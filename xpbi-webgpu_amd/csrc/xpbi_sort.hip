@@ -39,11 +39,26 @@ __global__ __launch_bounds__(256) void k_hash_count(const KP* __restrict__ kps, 
   forWork<kPersist>(W, [&](u32 scene, u32 first) {
     const KP& P = kps[scene];
     const u32 i = first + threadIdx.x;
-    if (i >= P.pEnd) return;
-    float4 p = pos[i];
-    u32 c = cellIndex(P, cellCoord(P, f3{p.x, p.y, p.z}));
-    keys[i] = c;
-    atomicAdd(&count[c], 1u);
+    const bool valid = i < P.pEnd;
+    u32 c = ~0u;
+    if (valid) {
+      float4 p = pos[i];
+      c = cellIndex(P, cellCoord(P, f3{p.x, p.y, p.z}));
+      keys[i] = c;
+    }
+    // the array is in the previous substep's cell order, so consecutive
+    // particles mostly share a cell: one atomic per run of equal cells within
+    // the wave (a run ends where the next lane's cell differs), adding the run's
+    // length.  The counts are the same integers in any order.
+    const u32 lane = threadIdx.x & 63u;
+    const u32 prev = __shfl_up(c, 1u, 64);
+    const bool head = lane == 0u || prev != c;
+    const uint64_t heads = __ballot(head);
+    if (valid && head) {
+      const uint64_t after = heads & ~((2ull << lane) - 1ull);  // heads of the lanes above (lane 63: none)
+      const u32 next = after ? (u32)__ffsll((unsigned long long)after) - 1u : 64u;
+      atomicAdd(&count[c], next - lane);
+    }
   });
 }
 

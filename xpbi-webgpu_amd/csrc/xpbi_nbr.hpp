@@ -91,6 +91,55 @@ __device__ __forceinline__ void forRowsQ(const KP& P, const u32* __restrict__ cs
   });
 }
 
+// forRowsQ with each candidate's work split in two: test(b, rec) in the walk,
+// and use(b, rec) only for the candidates the test accepts.  Each lane queues
+// its accepted candidates in LDS (column threadIdx.x of qbuf, kQueue deep) and
+// consumes them, reloading their records, in the lane's candidate order: when
+// some lane of the wave could not queue a further step, and once the walk
+// ends.  Calling use() inside the walk would run it for every candidate slot in
+// which any lane of the wave accepts, which is nearly every slot.
+constexpr u32 kQueue = 16;
+template <class Load, class Test, class Use>
+__device__ __forceinline__ void forRowsQQueued(const KP& P, const u32* __restrict__ cs, i3 cc, u32 q,
+                                               u32 (*qbuf)[256], Load&& load, Test&& test, Use&& use) {
+  const u32 tid = threadIdx.x;
+  u32 cnt = 0;
+  auto drain = [&]() {
+#pragma unroll 1
+    for (u32 k0 = 0; k0 < cnt; k0 += 4u) {
+      u32 b[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) b[j] = qbuf[k0 + (u32)j < cnt ? k0 + (u32)j : k0][tid];
+      decltype(load(0u)) d[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) d[j] = load(b[j]);
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        if (k0 + (u32)j < cnt) use(b[j], d[j]);
+    }
+    cnt = 0;
+  };
+  forRowSpans(P, cs, cc, [&](u32 beg, u32 end) {
+#pragma unroll 1
+    for (u32 b0 = beg + q; b0 < end; b0 += 16u) {
+      decltype(load(0u)) d[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++)
+        if (b0 + 4u * (u32)j < end) d[j] = load(b0 + 4u * (u32)j);
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const u32 b = b0 + 4u * (u32)j;
+        if (b < end && test(b, d[j])) {
+          qbuf[cnt][tid] = b;
+          cnt++;
+        }
+      }
+      if (__any(cnt > kQueue - 4u)) drain();  // the next step queues at most 4
+    }
+  });
+  drain();
+}
+
 // Support-neighbour list.  Entry k of particle i is lst[k * N + i], the u16
 // offset b - head[i].x of neighbour b from the particle's first candidate
 // (cellStart at the stencil's lowest cell (x-1, y-1, z-1), clamped to the grid):

@@ -409,6 +409,7 @@ __global__ __launch_bounds__(256) void k_pbfA(const KP* __restrict__ kps, WorkTa
   if (skipIteration(ctl, k)) return;
   __shared__ MatLds T;
   __shared__ float lds[PbfAAcc::kF][256];
+  __shared__ u32 qbuf[kQueue][256];
   u32 curScene = ~0u;
   forWork<kPersist>(W, [&](u32 scene, u32 first) {
     sceneMats(T, mats, scene, &curScene);
@@ -420,19 +421,18 @@ __global__ __launch_bounds__(256) void k_pbfA(const KP* __restrict__ kps, WorkTa
       const float* mRho = T.mRho[mi];  // m_b / rho0 by the neighbour's material
       const f3 xci = xyz(xc[i]);
       // candidates' predicted positions (x + dt v, tag) from the iteration's solveB
-      forRowsQ(P, cs, cellCoord(P, xyz(pi)), q, [&](u32 b) { return gat(xc, b); }, [&](u32 b, const float4& xb) {
-        if (b != i) {
-          const f3 dxv = xci - xyz(xb);
-          if (dist2(dxv) < P.supp2) {
-            const u32 kb = matOfTag(xb.w);
-            const float r = len(dxv);
-            acc.rho = acc.rho + T.mass[kb] * kernelPoly6(P, r);
-            const f3 g = kernelSpikyGradR(P, dxv, r) * mRho[kb];
-            acc.gs = acc.gs + g;
-            acc.g2 = acc.g2 + dot(g, g);
-          }
-        }
-      });
+      const float supp2 = P.supp2;
+      forRowsQQueued(P, cs, cellCoord(P, xyz(pi)), q, qbuf, [&](u32 b) { return gat(xc, b); },
+                     [&](u32 b, const float4& xb) { return b != i && dist2(xci - xyz(xb)) < supp2; },
+                     [&](u32 b, const float4& xb) {
+                       const f3 dxv = xci - xyz(xb);
+                       const u32 kb = matOfTag(xb.w);
+                       const float r = xsqrt(dist2(dxv));  // |dxv| from the support test's rounding
+                       acc.rho = acc.rho + T.mass[kb] * kernelPoly6(P, r);
+                       const f3 g = kernelSpikyGradR(P, dxv, r) * mRho[kb];
+                       acc.gs = acc.gs + g;
+                       acc.g2 = acc.g2 + dot(g, g);
+                     });
     });
     const u32 i = first + threadIdx.x;
     if (i >= P.pEnd) return;
@@ -456,6 +456,7 @@ __global__ __launch_bounds__(256) void k_pbfB(const KP* __restrict__ kps, WorkTa
   if (skipIteration(ctl, k)) return;
   __shared__ MatLds T;
   __shared__ float lds[F3Acc::kF][256];
+  __shared__ u32 qbuf[kQueue][256];
   u32 curScene = ~0u;
   forWork<kPersist>(W, [&](u32 scene, u32 first) {
     sceneMats(T, mats, scene, &curScene);
@@ -467,24 +468,23 @@ __global__ __launch_bounds__(256) void k_pbfB(const KP* __restrict__ kps, WorkTa
       if (!iAmWater && !(head[i].y & kNbrWater)) return;
       const f3 xci = xyz(xc[i]);
       const float li = aux[i].w;
-      forRowsQ(P, cs, cellCoord(P, xyz(pi)), q, [&](u32 b) { return XL{gat(xc, b), gatW(aux, b)}; },
-               [&](u32 b, const XL& r) {
-                 if (b != i) {
-                   const u32 mb = matOfTag(r.x.w);
-                   const bool bWater = !((T.solidMask >> mb) & 1u);
-                   if (iAmWater || bWater) {
-                     const f3 dxv = xci - xyz(r.x);
-                     if (dist2(dxv) < P.supp2) {
+      const float supp2 = P.supp2;
+      forRowsQQueued(P, cs, cellCoord(P, xyz(pi)), q, qbuf, [&](u32 b) { return XL{gat(xc, b), gatW(aux, b)}; },
+                     [&](u32 b, const XL& r) {
+                       const bool bWater = !((T.solidMask >> matOfTag(r.x.w)) & 1u);
+                       return b != i && (iAmWater || bWater) && dist2(xci - xyz(r.x)) < supp2;
+                     },
+                     [&](u32 b, const XL& r) {
+                       const u32 mb = matOfTag(r.x.w);
+                       const bool bWater = !((T.solidMask >> mb) & 1u);
+                       const f3 dxv = xci - xyz(r.x);
                        const u32 wk = bWater ? mb : mi;
-                       const float rr = len(dxv);
+                       const float rr = xsqrt(dist2(dxv));  // |dxv| from the support test's rounding
                        const float ratio = kernelPoly6(P, rr) * P.invW0;
                        const float r2 = ratio * ratio;
                        const float sCorr = -P.sCorrK * (r2 * r2);
                        acc.v = acc.v + (li + r.l + sCorr) * kernelSpikyGradR(P, dxv, rr) * T.mRho[wk][wk];
-                     }
-                   }
-                 }
-               });
+                     });
     });
     const u32 i = first + threadIdx.x;
     if (i >= P.pEnd) return;

@@ -291,16 +291,27 @@ def sand_column_1m(seed=1):
     return sand_column(80, 160, 80, 0.005, [1.5, 1.0, 1.5], seed)
 
 
-def dam_breach_1m(seed=3, dt=1.0 / 6000.0, iterations=10):
-    """Dam breach: 60x100x100 PBF water next to a 50x80x100 DP sand wall (phi 35), r = 0.005."""
+def dam_breach(scale=1.0, seed=3, dt=1.0 / 6000.0, iterations=10):
+    """Dam breach at r = 0.005: PBF water (rho0 1000, CFM eps 100, s_corr k 0.1) next to
+    a DP sand wall (phi 35), the blocks and the domain scaled by `scale` from the
+    1M layout (60x100x100 water, 50x80x100 sand in [0,1.5]x[0,0.75]x[0,0.5])."""
     r = 0.005
     s = _solver(r, dt, iterations, pbfEps=100.0, sCorrK=0.1)
-    s["domainMin"], s["domainMax"] = [0.0, 0.0, 0.0], [1.5, 0.75, 0.5]
+    s["domainMin"], s["domainMax"] = [0.0, 0.0, 0.0], [1.5 * scale, 0.75 * scale, 0.5 * scale]
     out, rng = _new(), Rng(seed, 0)
-    sample_box(out, [0.05, 0.0, 0.0], [0.05 + 60 * r, 100 * r, 100 * r], r, 1, rng, 0.1)
-    sample_box(out, [0.35, 0.0, 0.0], [0.35 + 50 * r, 80 * r, 100 * r], r, 0, rng, 0.1)
+    nw = [int(round(60 * scale)), int(round(100 * scale)), int(round(100 * scale))]
+    ns = [int(round(50 * scale)), int(round(80 * scale)), int(round(100 * scale))]
+    x0w, x0s = 0.05 * scale, 0.05 * scale + 60 * scale * r
+    sample_box(out, [x0w, 0.0, 0.0], [x0w + nw[0] * r, nw[1] * r, nw[2] * r], r, 1, rng, 0.1)
+    sample_box(out, [x0s, 0.0, 0.0], [x0s + ns[0] * r, ns[1] * r, ns[2] * r], r, 0, rng, 0.1)
     mats = [DP_SAND_STIFF(35.0), WATER(1000.0)]
-    return _finish(s, out, mats, seed, True, info="dam breach 600k water + 400k sand")
+    nW, nS = nw[0] * nw[1] * nw[2], ns[0] * ns[1] * ns[2]
+    return _finish(s, out, mats, seed, True, info=f"dam breach {nW} water + {nS} sand")
+
+
+def dam_breach_1m(seed=3, dt=1.0 / 6000.0, iterations=10):
+    """Dam breach: 60x100x100 PBF water next to a 50x80x100 DP sand wall (phi 35), r = 0.005."""
+    return dam_breach(1.0, seed, dt, iterations)
 
 
 def vm_poke(n, r, seed, dt=1.0 / 6000.0, iterations=20, sigY=5e3):

@@ -34,10 +34,12 @@ def moving_collider(scene, sim):
     return place
 
 
-def collect(scene, settle, substeps, tol=1e-4, cap=200, keep=0.25, seed=0, device=0):
+def collect(scene, settle, substeps, tol=1e-4, cap=200, keep=0.25, seed=0, device=0, max_strain=None):
     """Run `settle` plain substeps, then `substeps` more (plain Jacobi to `tol`),
     keeping a random fraction `keep` of each substep's active pairs.  A scene with a
-    moving collider gets its position updated every substep.
+    moving collider gets its position updated every substep.  With `max_strain`,
+    pairs whose trial or plastic strain features (100 log sigma, 100 tr, 100 |dev|)
+    exceed it in magnitude are dropped.
     Returns features (n x 12), targets (n x 2) and weights (n)."""
     sim = Simulation(scene, device=device)
     sim.writeParams(params_from_scene(scene))
@@ -59,7 +61,10 @@ def collect(scene, settle, substeps, tol=1e-4, cap=200, keep=0.25, seed=0, devic
         k += 1
         x, t, w = sim.collect_warmstart_pairs(cap, water)
         # the warm start clamps rho to [0, 2]: targets outside cannot be reached
-        ok = np.nonzero(np.isfinite(t[:, 0]) & (t[:, 0] >= 0.0) & (t[:, 0] <= 2.0) & (w > 0))[0]
+        good = np.isfinite(t[:, 0]) & (t[:, 0] >= 0.0) & (t[:, 0] <= 2.0) & (w > 0)
+        if max_strain is not None:
+            good &= np.all(np.abs(x[:, :5]) <= max_strain, axis=1)
+        ok = np.nonzero(good)[0]
         if keep < 1.0:
             ok = rng.choice(ok, size=max(1, int(len(ok) * keep)), replace=False) if len(ok) else ok
         xs.append(x[ok])
@@ -72,20 +77,28 @@ def collect(scene, settle, substeps, tol=1e-4, cap=200, keep=0.25, seed=0, devic
 def training_columns():
     """The training scenes -- (name, scene, settle substeps, collected substeps, kept
     fraction): DP sand columns (phi 30) collapsing on the floor at the two resolutions
-    of the 128k and 1M configurations, and a von Mises block poked by a moving sphere
-    (125k particles at r = 0.01, unlike the 1M poke benchmark), each with its own seed."""
+    of the 128k and 1M configurations, a von Mises block poked by a moving sphere
+    (125k particles at r = 0.01, unlike the 1M poke benchmark) and the dam breach at
+    half the 1M layout (its sand wall struck by the PBF water, s_corr k 0.1), each
+    with its own seed.  The water bursting under s_corr crushes parts of the sand
+    wall to the singular-value clamp; the dam breach keeps the pairs with strains
+    below 20 % (the columns' pairs stay below 10 %)."""
     return [("DP sand column 40x80x40 r=0.01 (128,000)", scenes.sand_column_128k(), 300, 40, 0.25),
             ("DP sand column 40x80x40 r=0.005 (128,000)",
              scenes.sand_column(40, 80, 40, 0.005, [0.75, 0.5, 0.75], seed=7), 420, 40, 0.25),
-            ("von Mises block 50^3 r=0.01 + poke (125,000)", scenes.vm_poke(50, 0.01, seed=9), 440, 40, 0.25)]
+            ("von Mises block 50^3 r=0.01 + poke (125,000)", scenes.vm_poke(50, 0.01, seed=9), 440, 40, 0.25),
+            ("dam breach 30x50x50 water + 25x40x50 sand r=0.005 (125,000)", scenes.dam_breach(0.5, seed=11), 60,
+             40, 0.25, 20.0)]
 
 
 def training_set(device=0, seed=0, log=None, plan=None):
     """Pairs of every training scene; each scene's weights are normalised to mean 1."""
     xs, ts, ws = [], [], []
-    for k, (name, sc, settle, n, keep) in enumerate(plan or training_columns()):
+    for k, entry in enumerate(plan or training_columns()):
+        name, sc, settle, n, keep = entry[:5]
         t0 = time.time()
-        x, t, w = collect(sc, settle, n, keep=keep, seed=seed + k, device=device)
+        x, t, w = collect(sc, settle, n, keep=keep, seed=seed + k, device=device,
+                          max_strain=entry[5] if len(entry) > 5 else None)
         if log:
             log(f"pairs: {name}: {len(x)} in {time.time() - t0:.1f} s")
         if len(w):

@@ -156,7 +156,8 @@ def trained_network(weights=None, log=None):
     X, T, W = warmstart.training_set(log=log)
     t1 = time.perf_counter()
     params, losses = warmstart.train(X, T, W, epochs=30)
-    return params, dict(scenes=[c[0] for c in warmstart.training_columns()], pairs=int(len(X)), epochs=30,
+    return params, dict(scenes=[c[0] for c in warmstart.training_columns()], pairs=int(len(X)), epochs=len(losses),
+                        schedule="30 epochs at lr 1e-3, 15 at 3e-4, 15 at 1e-4",
                         loss_first_epoch=float(losses[0]), loss_last_epoch=float(losses[-1]),
                         collect_s=round(t1 - t0, 2), train_s=round(time.perf_counter() - t1, 2))
 

@@ -120,6 +120,24 @@ def test_scene_generation_properties():
     assert abs(u.mean() - 0.5) < 0.005 and u.min() >= 0 and u.max() < 1
 
 
+def test_dam_breach_scales():
+    """The dam breach at scale 1 is the 1M layout (600k water beside 400k sand); at
+    scale 0.5 the blocks keep spacing r = 0.005 with half the lattice counts per axis
+    (30x50x50 water, 25x40x50 sand) in a domain of half the extent, side by side."""
+    full = scenes.dam_breach_1m()
+    assert full["positions"].shape[0] == 1_000_000
+    assert int((full["matIds"] == 1).sum()) == 600_000 and int((full["matIds"] == 0).sum()) == 400_000
+    half = scenes.dam_breach(0.5, seed=11)
+    p, m = half["positions"].astype(np.float64), half["matIds"]
+    assert int((m == 1).sum()) == 30 * 50 * 50 and int((m == 0).sum()) == 25 * 40 * 50
+    assert half["domainMax"] == [0.75, 0.375, 0.25]
+    assert (p >= 0).all() and (p <= np.array(half["domainMax"]) + 1e-6).all()
+    water, sand = p[m == 1], p[m == 0]
+    assert water[:, 0].max() < sand[:, 0].min() + 0.005  # water left of the sand wall
+    assert abs(np.ptp(sand[:, 1]) - 40 * 0.005) < 0.01 and abs(np.ptp(water[:, 1]) - 50 * 0.005) < 0.01
+    assert half["sCorrK"] == full["sCorrK"] == 0.1 and half["hasWater"]
+
+
 @pytest.mark.skipif(shutil.which("node") is None, reason="node not installed")
 def test_napi_addon_loads():
     out = subprocess.check_output(["node", os.path.join(JS, "run.js"), "exports"], text=True)

@@ -211,21 +211,31 @@ __device__ __forceinline__ f3 kernelGradW(const KP& P, f3 dxv) {
 // so grad W = dxv * gradC * t^3 needs one sqrt and no division.  r2 = |dxv|^2 is
 // already known (< support^2) from the neighbour test; the r < 1e-9 case of the
 // reference returns zero.
-__device__ __forceinline__ f3 gradWfast(const KP& P, f3 dxv, float r2) {
+// These kernels take the scene record KP or its register copy SphC (the same
+// field names): read through the record inside a neighbour loop, the constants
+// are reloaded from memory at every neighbour where scalar registers run short.
+template <class K> __device__ __forceinline__ f3 gradWfast(const K& P, f3 dxv, float r2) {
   float r = fsqrt(r2);
   float t = 1.0f - r * P.halfInvH;
   float c = (r2 < 1e-18f) ? 0.0f : P.gradC * (t * t * t);
   return dxv * c;
 }
-__device__ __forceinline__ float kernelWfast(const KP& P, float r2) {
+template <class K> __device__ __forceinline__ float kernelWfast(const K& P, float r2) {
   float q = fsqrt(r2) * P.invH;
   float t = 1.0f - 0.5f * q;
   float t2 = t * t;
   return P.sigW * (t2 * t2) * (2.0f * q + 1.0f);
 }
+// the scene's SPH constants of the neighbour sweeps, copied into registers
+struct SphC {
+  float halfInvH, gradC, invH, sigW, support, poly6C, spikyC, invW0, sCorrK, dt, minD;
+};
+__device__ __forceinline__ SphC sphConsts(const KP& P) {
+  return SphC{P.halfInvH, P.gradC, P.invH, P.sigW, P.support, P.poly6C, P.spikyC, P.invW0, P.sCorrK, P.dt, P.minD};
+}
 
 // common.js:82-87
-__device__ __forceinline__ float kernelPoly6(const KP& P, float r) {
+template <class K> __device__ __forceinline__ float kernelPoly6(const K& P, float r) {
   float H = P.support;
   if (r >= H) return 0.0f;
   float x = H * H - r * r;
@@ -233,7 +243,7 @@ __device__ __forceinline__ float kernelPoly6(const KP& P, float r) {
 }
 // common.js:88-94 with the pair distance r = |dxv| already formed and the
 // division by r as a product with the hardware reciprocal (1 ulp)
-__device__ __forceinline__ f3 kernelSpikyGradR(const KP& P, f3 dxv, float r) {
+template <class K> __device__ __forceinline__ f3 kernelSpikyGradR(const K& P, f3 dxv, float r) {
   float H = P.support;
   if (r >= H || r < 1e-9f) return splat3(0.0f);
   float t = H - r;

@@ -747,7 +747,7 @@ struct SpLate {
 // pair's velocity is not loaded and the test is not evaluated.  The bound
 // carries a margin above the rounding of the f32 test, so the skipped pairs are
 // exactly those the test rejects.
-__device__ __forceinline__ void solveBPair(const KP& P, float Vi, const float* wc_i, f3 xci, f3 dxv, float r2,
+__device__ __forceinline__ void solveBPair(const SphC& P, float Vi, const float* wc_i, f3 xci, f3 dxv, float r2,
                                            const f33& Spb, bool near, f3 xb, float4 vb, float tagB, f3* dv,
                                            f3* corr) {
   const f3 dxp = -dxv;  // x_b - x_i
@@ -808,22 +808,23 @@ __device__ __forceinline__ void solveBGatherQ(const KP& P, const MatLds& T, u32 
   const f3 xi = xyz(pi);
   const f3 xci = xi + P.dt * xyz(vi4);
   const float* wc_i = T.wc[matOfTag(pi.w)];
+  const SphC K = sphConsts(P);
   if constexpr (kAllSolid) {
     const float rc2 = contactBound2(P, xi, xyz(vi4), vmax);
     forSupportLate<kL>(P, cs, pos, nl, i, xi, q,
-        [&](u32 b) { return SpGather{Sp.cx[b], Sp.a[b], Sp.b[b]}; },
+        [&](u32 b) { return SpGather{gat(Sp.cx, b), gat(Sp.a, b), gat(Sp.b, b)}; },
         [&](u32 b, const SpGather& g, float r2, bool valid) {
           SpLate e{make_float4(0.0f, 0.0f, 0.0f, 0.0f), 0.0f, false};
           if (valid && !(r2 >= rc2)) {
-            e.v = vin[b];
-            e.tag = pos[b].w;
+            e.v = gat(vin, b);
+            e.tag = gatW(pos, b);
             e.near = true;
           }
           return e;
         },
         [&](u32 b, const SpGather& g, const SpLate& e, f3 dxv, float r2) {
           const f33 Spb{{f3{g.a.x, g.a.y, g.a.z}, f3{g.a.w, g.b.x, g.b.y}, f3{g.b.z, g.b.w, g.p.w}}};
-          solveBPair(P, Vi, wc_i, xci, dxv, r2, Spb, e.near, xyz(g.p), e.v, e.tag, &a.dv, &a.corr);
+          solveBPair(K, Vi, wc_i, xci, dxv, r2, Spb, e.near, xyz(g.p), e.v, e.tag, &a.dv, &a.corr);
         });
   } else {
     const float* __restrict__ c9 = (const float*)Sp.cx + 3;
@@ -832,7 +833,7 @@ __device__ __forceinline__ void solveBGatherQ(const KP& P, const MatLds& T, u32 
         [&](u32 b, const SpGatherW& g, f3 dxv, float r2) {
           if (!solidTag(T.solidMask, g.p.w)) return;
           const f33 Spb{{f3{g.a.x, g.a.y, g.a.z}, f3{g.a.w, g.b.x, g.b.y}, f3{g.b.z, g.b.w, g.c}}};
-          solveBPair(P, Vi, wc_i, xci, dxv, r2, Spb, true, xyz(g.p), g.v, g.p.w, &a.dv, &a.corr);
+          solveBPair(K, Vi, wc_i, xci, dxv, r2, Spb, true, xyz(g.p), g.v, g.p.w, &a.dv, &a.corr);
         });
   }
 }

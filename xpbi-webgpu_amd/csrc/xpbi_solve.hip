@@ -422,14 +422,15 @@ __global__ __launch_bounds__(256) void k_pbfA(const KP* __restrict__ kps, WorkTa
       const f3 xci = xyz(xc[i]);
       // candidates' predicted positions (x + dt v, tag) from the iteration's solveB
       const float supp2 = P.supp2;
+      const SphC K = sphConsts(P);
       forRowsQQueued(P, cs, cellCoord(P, xyz(pi)), q, qbuf, [&](u32 b) { return gat(xc, b); },
                      [&](u32 b, const float4& xb) { return b != i && dist2(xci - xyz(xb)) < supp2; },
                      [&](u32 b, const float4& xb) {
                        const f3 dxv = xci - xyz(xb);
                        const u32 kb = matOfTag(xb.w);
                        const float r = xsqrt(dist2(dxv));  // |dxv| from the support test's rounding
-                       acc.rho = acc.rho + T.mass[kb] * kernelPoly6(P, r);
-                       const f3 g = kernelSpikyGradR(P, dxv, r) * mRho[kb];
+                       acc.rho = acc.rho + T.mass[kb] * kernelPoly6(K, r);
+                       const f3 g = kernelSpikyGradR(K, dxv, r) * mRho[kb];
                        acc.gs = acc.gs + g;
                        acc.g2 = acc.g2 + dot(g, g);
                      });
@@ -469,6 +470,7 @@ __global__ __launch_bounds__(256) void k_pbfB(const KP* __restrict__ kps, WorkTa
       const f3 xci = xyz(xc[i]);
       const float li = aux[i].w;
       const float supp2 = P.supp2;
+      const SphC K = sphConsts(P);
       forRowsQQueued(P, cs, cellCoord(P, xyz(pi)), q, qbuf, [&](u32 b) { return XL{gat(xc, b), gatW(aux, b)}; },
                      [&](u32 b, const XL& r) {
                        const bool bWater = !((T.solidMask >> matOfTag(r.x.w)) & 1u);
@@ -480,10 +482,10 @@ __global__ __launch_bounds__(256) void k_pbfB(const KP* __restrict__ kps, WorkTa
                        const f3 dxv = xci - xyz(r.x);
                        const u32 wk = bWater ? mb : mi;
                        const float rr = xsqrt(dist2(dxv));  // |dxv| from the support test's rounding
-                       const float ratio = kernelPoly6(P, rr) * P.invW0;
+                       const float ratio = kernelPoly6(K, rr) * K.invW0;
                        const float r2 = ratio * ratio;
-                       const float sCorr = -P.sCorrK * (r2 * r2);
-                       acc.v = acc.v + (li + r.l + sCorr) * kernelSpikyGradR(P, dxv, rr) * T.mRho[wk][wk];
+                       const float sCorr = -K.sCorrK * (r2 * r2);
+                       acc.v = acc.v + (li + r.l + sCorr) * kernelSpikyGradR(K, dxv, rr) * T.mRho[wk][wk];
                      });
     });
     const u32 i = first + threadIdx.x;
@@ -511,13 +513,14 @@ __global__ __launch_bounds__(256) void k_xsph(const KP* __restrict__ kps, WorkTa
       const float4 pi = pos[i];
       const f3 vi = xyz(vin[i]);
       const bool iSolid = solidTag(T.solidMask, pi.w);
+      const SphC K = sphConsts(P);
       // kAll: no water in the scene, every neighbour shares the particle's phase
       auto gather = [&](auto all) {
         constexpr bool kAll = decltype(all)::value;
         forSupport<kLanesX>(P, cs, pos, nl, i, xyz(pi), q, [&](u32 b) { return PV{gat(pos, b), gat(vin, b)}; },
                     [&](u32 b, const PV& r, f3 dxv, float r2) {
                       if (kAll || solidTag(T.solidMask, r.p.w) == iSolid)
-                        acc.v = acc.v + r.v.w * (xyz(r.v) - vi) * kernelWfast(P, r2);
+                        acc.v = acc.v + r.v.w * (xyz(r.v) - vi) * kernelWfast(K, r2);
                     });
       };
       if (allSolid(T)) gather(std::true_type{});

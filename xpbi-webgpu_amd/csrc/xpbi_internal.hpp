@@ -191,7 +191,6 @@ size_t radix_hist_needed(u64 n);
 int radix_passes(u32 C);
 void launch_solve_stages(Sim* s, int iterations, bool water, int stop);
 void launch_copy(Sim* s, const float4* src, float4* dst, u64 n);
-void launch_mlp_apply(Sim* s);
 void mlp_pack_host(const float* params, void* packed);
 size_t mlp_packed_bytes();
 

@@ -48,10 +48,11 @@ __device__ __forceinline__ void load_x8(const float* __restrict__ X, u64 row, bo
 }
 
 // ---------------------------------------------------------------- inference
-__global__ __launch_bounds__(kMlpBlock) void k_mlp_predict(const MlpPacked* __restrict__ Wg, const float* __restrict__ X,
+__global__ __launch_bounds__(kMlpBlock) void k_mlp_predict(const MlpFwd* __restrict__ Wg, const float* __restrict__ X,
                                                            u64 rows, float* __restrict__ Y) {
-  __shared__ MlpPacked W;
-  stage_weights(W, Wg);
+  __shared__ MlpFwd W;
+  mlp_stage_fwd(W, Wg);
+  __syncthreads();
   const u32 l = threadIdx.x & 63u, w = threadIdx.x >> 6;
   const u64 base = ((u64)blockIdx.x * kMlpWaves + w) * 32u;
   const u64 n = base + (l & 31u);
@@ -97,7 +98,7 @@ __global__ __launch_bounds__(kMlpBlock) void k_mlp_train(const MlpPacked* __rest
     float x8[8];
     load_x8(X, row, valid, x8);
     MlpAct a;
-    mlp_forward(W, x8, a);
+    mlp_forward(W.f, x8, a);
     // output error
     float dy[2];
     const float wn = (valid && Wt) ? Wt[row] : 1.0f;
@@ -116,7 +117,7 @@ __global__ __launch_bounds__(kMlpBlock) void k_mlp_train(const MlpPacked* __rest
 #pragma unroll
       for (int r = 0; r < 16; r++) {
         const int u = mlp_dunit(to, r, h);
-        float v = W.w3[u] * dy[0] + W.w3[kMlpH + u] * dy[1];
+        float v = W.f.w3[u] * dy[0] + W.f.w3[kMlpH + u] * dy[1];
         if (!((a.pos2 >> (16 * to + r)) & 1u)) v = 0.0f;
         dz2b[to][r / 8][r % 8] = to_bf16(v);
         st.dz2[u * 32 + n32] = bf16_bits(v);
@@ -263,47 +264,7 @@ __global__ __launch_bounds__(256) void k_mlp_pack(const float* __restrict__ w, M
   if (k < (u32)kMlpParams) mlp_pack_one(P, (int)k, w[k]);
 }
 
-// ---------------------------------------------------------------- warm start in the substep
-// lambda0 = clamp(y0, 0, 2) d0 per particle (d0 = feature row slot 12, zero for
-// particles without an active constraint; fmaxf maps a NaN y0 to 0, so such a
-// particle starts plain Jacobi from zero); the unscaled Sp / selfDv written by
-// the features kernel are scaled by lambda0, and y1 is the HB bracket position.
-// One chunk of 32 kMlpWaves particles per block: a grid-strided variant that
-// staged the weights once per block measured slower (85 against 60 us at 1M
-// particles), its chunks' loads no longer overlapping across blocks.
-__global__ __launch_bounds__(kMlpBlock) void k_mlp_apply(const MlpPacked* __restrict__ Wg, const float* __restrict__ X,
-                                                         u32 N, float4* __restrict__ aux, SpP Sp,
-                                                         float4* __restrict__ selfDv, float* __restrict__ hbT) {
-  __shared__ MlpPacked W;
-  stage_weights(W, Wg);
-  const u32 l = threadIdx.x & 63u, w = threadIdx.x >> 6;
-  const u32 n = (blockIdx.x * kMlpWaves + w) * 32u + (l & 31u);
-  float x8[8];
-  load_x8(X, n, n < N, x8);
-  MlpAct a;
-  mlp_forward(W, x8, a);
-  if (l < 32u && n < N) {
-    const float d0 = X[(u64)n * kFeatStride + 12];
-    const float lam = d0 == 0.0f ? 0.0f : fminf(fmaxf(a.y[0], 0.0f), 2.0f) * d0;
-    aux[n].y = lam;
-    float4 A = Sp.a[n], B = Sp.b[n];
-    Sp.a[n] = make_float4(A.x * lam, A.y * lam, A.z * lam, A.w * lam);
-    Sp.b[n] = make_float4(B.x * lam, B.y * lam, B.z * lam, B.w * lam);
-    Sp.cx[n].w = Sp.cx[n].w * lam;
-    const float4 sd = selfDv[n];
-    selfDv[n] = make_float4(sd.x * lam, sd.y * lam, sd.z * lam, 0.0f);
-    if (hbT) hbT[n] = a.y[1];
-  }
-}
-
 static inline u32 cdivu(u64 a, u64 b) { return (u32)((a + b - 1) / b); }
-
-void launch_mlp_apply(Sim* s) {
-  if (s->N == 0) return;
-  hipLaunchKernelGGL(k_mlp_apply, dim3(cdivu(s->N, 32 * kMlpWaves)), dim3(kMlpBlock), 0, s->stream,
-                     (const MlpPacked*)s->wsPacked, (const float*)s->wsFeat, s->N, s->set[s->cur].aux, s->Sp,
-                     s->selfDv, s->wsHb);
-}
 
 // host-side packing of canonical parameters (the sim's copy)
 void mlp_pack_host(const float* params, void* packed) {
@@ -507,7 +468,7 @@ int xpbi_mlp_predict(xpbi_mlp* m, const float* x, size_t rows, float* y) {
   hipError_t e = hipMalloc((void**)&dY, rows * 2 * sizeof(float));
   if (e == hipSuccess) {
     hipLaunchKernelGGL(k_mlp_predict, dim3(cdivu(rows, 32 * kMlpWaves)), dim3(kMlpBlock), 0, m->stream,
-                       (const MlpPacked*)m->P, (const float*)dX, (u64)rows, dY);
+                       (const MlpFwd*)m->P, (const float*)dX, (u64)rows, dY);
     e = hipStreamSynchronize(m->stream);
   }
   if (e == hipSuccess) e = hipMemcpy(y, dY, rows * 2 * sizeof(float), hipMemcpyDeviceToHost);

@@ -53,13 +53,17 @@ __host__ __device__ inline float bf16_float(uint16_t b) {
 }
 __host__ __device__ inline float bf16_round(float f) { return bf16_float(bf16_bits(f)); }
 
-// packed bf16 weights of one network (what a kernel reads)
-struct alignas(16) MlpPacked {
+// packed bf16 weights of one network (what a kernel reads): the forward part,
+// which inference stages alone, then the transposed layer 2 of the backward pass
+struct alignas(16) MlpFwd {
   uint16_t w1[kMlpH * 16];     // [unit][feature], features 12..15 zero
   uint16_t w2f[kMlpH * kMlpH];  // forward A operand of layer 2, [t'][t][s][lane][j]
-  uint16_t w2b[kMlpH * kMlpH];  // A operand of dh1 = W2^T dz2, [ti][t'][s][lane][j]
   float w3[kMlpOut * kMlpH];    // bf16-rounded, [k][unit]
   float b1[kMlpH], b2[kMlpH], b3[4];
+};
+struct alignas(16) MlpPacked {
+  MlpFwd f;
+  uint16_t w2b[kMlpH * kMlpH];  // A operand of dh1 = W2^T dz2, [ti][t'][s][lane][j]
 };
 
 // unit of register j (< 8) of lane half h in K-step s of 32-unit tile t
@@ -87,21 +91,21 @@ __host__ __device__ inline void mlp_w2_slots(int o, int i, int* fwd, int* bwd) {
 __host__ __device__ inline void mlp_pack_one(MlpPacked* P, int k, float v) {
   if (k < kOffB1) {
     const int u = k / kMlpIn, f = k % kMlpIn;
-    P->w1[u * 16 + f] = bf16_bits(v);
+    P->f.w1[u * 16 + f] = bf16_bits(v);
   } else if (k < kOffW2) {
-    P->b1[k - kOffB1] = v;
+    P->f.b1[k - kOffB1] = v;
   } else if (k < kOffB2) {
     const int q = k - kOffW2, o = q / kMlpH, i = q % kMlpH;
     int a, b;
     mlp_w2_slots(o, i, &a, &b);
-    P->w2f[a] = bf16_bits(v);
+    P->f.w2f[a] = bf16_bits(v);
     P->w2b[b] = bf16_bits(v);
   } else if (k < kOffW3) {
-    P->b2[k - kOffB2] = v;
+    P->f.b2[k - kOffB2] = v;
   } else if (k < kOffB3) {
-    P->w3[k - kOffW3] = bf16_round(v);
+    P->f.w3[k - kOffW3] = bf16_round(v);
   } else {
-    P->b3[k - kOffB3] = v;
+    P->f.b3[k - kOffB3] = v;
   }
 }
 
@@ -135,11 +139,10 @@ struct MlpAct {
   float y[2];
 };
 
-// forward of the wave's 32 samples; x8 = this lane's 8 features (8h..8h+7, zero past 11)
-__device__ __forceinline__ void mlp_forward(const MlpPacked& W, const float x8[8], MlpAct& a) {
+// forward of the wave's 32 samples; xb = this lane's 8 features (8h..8h+7, zero past 11)
+__device__ __forceinline__ void mlp_forward_b(const MlpFwd& W, bf16x8 xb, MlpAct& a) {
   const int l = threadIdx.x & 63, h = l >> 5;
-#pragma unroll
-  for (int j = 0; j < 8; j++) a.xb[j] = to_bf16(x8[j]);
+  a.xb = xb;
   a.pos1 = 0u;
   a.pos2 = 0u;
 #pragma unroll
@@ -181,6 +184,19 @@ __device__ __forceinline__ void mlp_forward(const MlpPacked& W, const float x8[8
     acc += __shfl_xor(acc, 32, 64);
     a.y[k] = acc + W.b3[k];
   }
+}
+// the same with the features as f32
+__device__ __forceinline__ void mlp_forward(const MlpFwd& W, const float x8[8], MlpAct& a) {
+  bf16x8 xb;
+#pragma unroll
+  for (int j = 0; j < 8; j++) xb[j] = to_bf16(x8[j]);
+  mlp_forward_b(W, xb, a);
+}
+// stage the forward weights into LDS (the caller synchronises the block)
+__device__ __forceinline__ void mlp_stage_fwd(MlpFwd& s, const MlpFwd* __restrict__ g) {
+  const uint4* src = (const uint4*)g;
+  uint4* dst = (uint4*)&s;
+  for (uint32_t k = threadIdx.x; k < sizeof(MlpFwd) / 16; k += blockDim.x) dst[k] = src[k];
 }
 #endif
 

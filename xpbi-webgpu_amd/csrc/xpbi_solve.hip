@@ -603,14 +603,29 @@ __global__ __launch_bounds__(256) void k_integrate(const KP* __restrict__ kps, W
 // 13 HB root position of the return map (NaN: none), 14 C, 15 the first step's
 // numerator -C - gamma dt dC.v (its square weighs the particle in the residual).
 // Also reduces sum C^2 (the residual's denominator) per block.
-template <bool kPersist>
-__global__ __launch_bounds__(256) void k_features(const KP* __restrict__ kps, WorkTab W, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
-                                                  const float4* __restrict__ vin, M3C Fm, M3C Lm,
-                                                  const float4* __restrict__ aux, SpP Sp, float4* __restrict__ selfDv,
-                                                  const u32* __restrict__ cs, NLc nl,
-                                                  const float* __restrict__ lamPrev, float* __restrict__ feat,
-                                                  double* __restrict__ part, u32* __restrict__ vmaxOut) {
+//
+// kMlp: the learned first step in the same pass.  Each wave runs the network
+// (xpbi_mlp.hpp) on its 64 particles as two MFMA passes of 32, the bf16 features
+// fetched across lanes; lane l keeps the output of pass l / 32, which is its own
+// particle.  Then lambda0 = clamp(y0, 0, 2) d0 (0 without an active constraint)
+// goes to aux.y, the stored Sp and selfDv are lambda0 times the unscaled ones, and
+// y1 is the HB bracket position of the next solveA; the feature rows are not
+// written.  lambda0 is NaN-safe: fmaxf maps a NaN y0 to 0, so such a particle
+// starts plain Jacobi from zero.
+template <bool kPersist, bool kMlp>
+__device__ __forceinline__ void featuresBody(const KP* __restrict__ kps, const WorkTab& W, const DevMat* __restrict__ mats,
+                                             const float4* __restrict__ pos, const float4* __restrict__ vin, M3C Fm,
+                                             M3C Lm, float4* __restrict__ aux, SpP Sp, float4* __restrict__ selfDv,
+                                             const u32* __restrict__ cs, const NLc& nl,
+                                             const float* __restrict__ lamPrev, float* __restrict__ feat,
+                                             const MlpFwd* __restrict__ Wn, float* __restrict__ hbT,
+                                             double* __restrict__ part, u32* __restrict__ vmaxOut) {
   __shared__ MatLds T;
+  __shared__ std::conditional_t<kMlp, MlpFwd, char> Wl;
+  if constexpr (kMlp) {
+    mlp_stage_fwd(Wl, Wn);
+    __syncthreads();
+  }
   u32 curScene = ~0u;
   float c2 = 0.0f;
   SpeedBound sb;
@@ -620,69 +635,134 @@ __global__ __launch_bounds__(256) void k_features(const KP* __restrict__ kps, Wo
     const Moments mo = runMoments<kLanesA, true>(P, T, first, cs, pos, vin, nl);
     const u32 i = first + threadIdx.x;
     sb.note(scene);
-    if (i >= P.pEnd) return;
-    sb.add(scene, xyz(vin[i]));
+    const bool live = i < P.pEnd;
+    if (!kMlp && !live) return;
     float row[16];
 #pragma unroll
     for (int k = 0; k < 16; k++) row[k] = 0.0f;
     row[13] = __builtin_nanf("");
-    const float4 pi = pos[i];
-    const u32 mi = matOfTag(pi.w);
-    bool wrote = false;
-    if ((T.solidMask >> mi) & 1u) {
-      const float4 a = aux[i];
-      const Constraint c = evalConstraint(P, T, i, pi, a, mo, Fm, Lm, -1.0f, true);
-      if (c.active) {
-        const MatV m = hardenedMat(matv(T, mi), a.z);
-        const float g = P.kDamp;
-        const float den = (1.0f + g) * c.denom + P.alphaT;
-        const float d0 = (-c.C - g * P.dt * c.cdot) / den;
-        const f3 lt = vlog(safeSig(c.sigTrial));
-        const f3 lp = vlog(safeSig(c.sigPost));
-        const float trp = lp.x + lp.y + lp.z;
-        float strength = 0.0f;
-        switch (m.kind) {
-          case MAT_DP: strength = m.p0; break;
-          case MAT_VM:
-          case MAT_HB: strength = 100.0f * m.p0 / m.mu; break;
-          case MAT_NACC: strength = 0.1f * m.p2; break;
-          case MAT_SNOW: strength = 10.0f * m.p0; break;
-          default: break;
+    f33 gl = mzero<float>();
+    f3 sg = f3{0.0f, 0.0f, 0.0f};
+    float4 pi = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (live) {
+      sb.add(scene, xyz(vin[i]));
+      pi = pos[i];
+      const u32 mi = matOfTag(pi.w);
+      if ((T.solidMask >> mi) & 1u) {
+        const float4 a = aux[i];
+        const Constraint c = evalConstraint(P, T, i, pi, a, mo, Fm, Lm, -1.0f, true);
+        if (c.active) {
+          const MatV m = hardenedMat(matv(T, mi), a.z);
+          const float g = P.kDamp;
+          const float den = (1.0f + g) * c.denom + P.alphaT;
+          const float d0 = (-c.C - g * P.dt * c.cdot) / den;
+          const f3 lt = vlog(safeSig(c.sigTrial));
+          const f3 lp = vlog(safeSig(c.sigPost));
+          const float trp = lp.x + lp.y + lp.z;
+          float strength = 0.0f;
+          switch (m.kind) {
+            case MAT_DP: strength = m.p0; break;
+            case MAT_VM:
+            case MAT_HB: strength = 100.0f * m.p0 / m.mu; break;
+            case MAT_NACC: strength = 0.1f * m.p2; break;
+            case MAT_SNOW: strength = 10.0f * m.p0; break;
+            default: break;
+          }
+          row[0] = 100.0f * lt.x;
+          row[1] = 100.0f * lt.y;
+          row[2] = 100.0f * lt.z;
+          row[3] = 100.0f * trp;
+          row[4] = 100.0f * len(lp - splat3(trp / 3.0f));
+          row[5] = log10f(m.mu) - 5.0f;
+          row[6] = m.lam / (m.lam + 2.0f * m.mu);
+          row[7] = (float)m.kind / 6.0f;
+          row[8] = strength;
+          row[9] = (float)(nl.head[i].y & ~(kNbrFlag | kNbrWater)) / 40.0f;
+          row[10] = (d0 != 0.0f && lamPrev) ? wclamp(lamPrev[i] / d0, -8.0f, 8.0f) : 0.0f;
+          row[11] = P.alphaT / den;
+          row[12] = d0;
+          row[13] = c.hbT;
+          row[14] = c.C;
+          row[15] = -c.C - g * P.dt * c.cdot;
+          gl = c.GL;
+          sg = c.sgrad;
+          c2 += c.C * c.C;  // a persistent grid's thread walks several runs
         }
-        row[0] = 100.0f * lt.x;
-        row[1] = 100.0f * lt.y;
-        row[2] = 100.0f * lt.z;
-        row[3] = 100.0f * trp;
-        row[4] = 100.0f * len(lp - splat3(trp / 3.0f));
-        row[5] = log10f(m.mu) - 5.0f;
-        row[6] = m.lam / (m.lam + 2.0f * m.mu);
-        row[7] = (float)m.kind / 6.0f;
-        row[8] = strength;
-        row[9] = (float)(nl.head[i].y & ~(kNbrFlag | kNbrWater)) / 40.0f;
-        row[10] = (d0 != 0.0f && lamPrev) ? wclamp(lamPrev[i] / d0, -8.0f, 8.0f) : 0.0f;
-        row[11] = P.alphaT / den;
-        row[12] = d0;
-        row[13] = c.hbT;
-        row[14] = c.C;
-        row[15] = -c.C - g * P.dt * c.cdot;
-        stsp(Sp, i, c.GL, pi);
-        selfDv[i] = make_float4(c.sgrad.x, c.sgrad.y, c.sgrad.z, 0.0f);
-        c2 += c.C * c.C;  // a persistent grid's thread walks several runs
-        wrote = true;
       }
     }
-    if (!wrote) {
-      stsp(Sp, i, mzero<float>(), pi);
-      selfDv[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if constexpr (!kMlp) {
+      stsp(Sp, i, gl, pi);
+      selfDv[i] = make_float4(sg.x, sg.y, sg.z, 0.0f);
+      float4* o = (float4*)(feat + (u64)i * kFeatStride);
+      o[0] = make_float4(row[0], row[1], row[2], row[3]);
+      o[1] = make_float4(row[4], row[5], row[6], row[7]);
+      o[2] = make_float4(row[8], row[9], row[10], row[11]);
+      o[3] = make_float4(row[12], row[13], row[14], row[15]);
+    } else {
+      // features 0..11 as bf16 pairs; lane l of pass p takes the features 8h..8h+7
+      // (h = l / 32, zero past 11) of the pass's sample l % 32, lane 32p + l % 32
+      u32 xw[6];
+#pragma unroll
+      for (int k = 0; k < 6; k++) xw[k] = (u32)bf16_bits(row[2 * k]) | ((u32)bf16_bits(row[2 * k + 1]) << 16);
+      const u32 l = threadIdx.x & 63u, hh = l >> 5;
+      float y0 = 0.0f, y1 = 0.0f;
+#pragma unroll 1
+      for (u32 pass = 0; pass < 2u; pass++) {
+        const int src = (int)(32u * pass + (l & 31u));
+        u32 v[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) v[k] = (u32)__shfl((int)xw[k], src, 64);
+        const u32 q[4] = {hh ? v[4] : v[0], hh ? v[5] : v[1], hh ? 0u : v[2], hh ? 0u : v[3]};
+        bf16x8 xb;
+        __builtin_memcpy(&xb, q, 16);
+        u32 off = 0u;
+        asm volatile("" : "+v"(off));  // keeps the weight reads in the loop: hoisted, they take ~130 registers
+        MlpAct act;
+        mlp_forward_b(*(const MlpFwd*)((const char*)&Wl + off), xb, act);
+        if (hh == pass) {
+          y0 = act.y[0];
+          y1 = act.y[1];
+        }
+      }
+      if (live) {
+        const float d0 = row[12];
+        const float lam = d0 == 0.0f ? 0.0f : fminf(fmaxf(y0, 0.0f), 2.0f) * d0;
+        aux[i].y = lam;
+        Sp.a[i] = make_float4(gl.c[0].x * lam, gl.c[0].y * lam, gl.c[0].z * lam, gl.c[1].x * lam);
+        Sp.b[i] = make_float4(gl.c[1].y * lam, gl.c[1].z * lam, gl.c[2].x * lam, gl.c[2].y * lam);
+        Sp.cx[i] = make_float4(pi.x, pi.y, pi.z, gl.c[2].z * lam);
+        selfDv[i] = make_float4(sg.x * lam, sg.y * lam, sg.z * lam, 0.0f);
+        hbT[i] = y1;
+      }
     }
-    float4* o = (float4*)(feat + (u64)i * kFeatStride);
-    o[0] = make_float4(row[0], row[1], row[2], row[3]);
-    o[1] = make_float4(row[4], row[5], row[6], row[7]);
-    o[2] = make_float4(row[8], row[9], row[10], row[11]);
-    o[3] = make_float4(row[12], row[13], row[14], row[15]);
   });
   sb.raise(vmaxOut);
   blockReduce2(0.0, (double)c2, part + 2 * blockIdx.x);
+}
+
+template <bool kPersist>
+__global__ __launch_bounds__(256) void k_features(const KP* __restrict__ kps, WorkTab W, const DevMat* __restrict__ mats, const float4* __restrict__ pos,
+                                                  const float4* __restrict__ vin, M3C Fm, M3C Lm,
+                                                  float4* __restrict__ aux, SpP Sp, float4* __restrict__ selfDv,
+                                                  const u32* __restrict__ cs, NLc nl,
+                                                  const float* __restrict__ lamPrev, float* __restrict__ feat,
+                                                  double* __restrict__ part, u32* __restrict__ vmaxOut) {
+  featuresBody<kPersist, false>(kps, W, mats, pos, vin, Fm, Lm, aux, Sp, selfDv, cs, nl, lamPrev, feat, nullptr,
+                                nullptr, part, vmaxOut);
+}
+
+// the occupancy of k_features (4 waves per SIMD, 3 for the persistent loop): the
+// network's accumulators then live in VGPRs, without spills
+template <bool kPersist>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kPersist ? 3 : 4))) void k_features_mlp(const KP* __restrict__ kps, WorkTab W, const DevMat* __restrict__ mats,
+                                                      const float4* __restrict__ pos, const float4* __restrict__ vin,
+                                                      M3C Fm, M3C Lm, float4* __restrict__ aux, SpP Sp,
+                                                      float4* __restrict__ selfDv, const u32* __restrict__ cs, NLc nl,
+                                                      const float* __restrict__ lamPrev, const MlpFwd* __restrict__ Wn,
+                                                      float* __restrict__ hbT, double* __restrict__ part,
+                                                      u32* __restrict__ vmaxOut) {
+  featuresBody<kPersist, true>(kps, W, mats, pos, vin, Fm, Lm, aux, Sp, selfDv, cs, nl, lamPrev, nullptr, Wn, hbT,
+                               part, vmaxOut);
 }
 
 // sum C^2 of the features pass becomes the residual's denominator
@@ -796,22 +876,29 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
   const float* hbHint = s->wsMode == 1 ? (const float*)s->wsHb : nullptr;
   int warm = 0;  // the learned first iteration ran
   if (s->wsMode != 0) {
-    XPBI_LAUNCH_W(k_features, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
-                       (const float4*)s->vel[vi], cm(F), cm(s->L), (const float4*)S.aux, s->Sp, s->selfDv,
-                       (const u32*)s->cellStart, nl, (const float*)s->lamPrev, s->wsFeat, s->residPart, slotW());
+    // the learned first step: the network runs inside the features pass
+    const bool fused = s->wsMode == 1 && iterations > 0;
+    if (fused) {
+      XPBI_LAUNCH_W(k_features_mlp, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
+                         (const float4*)s->vel[vi], cm(F), cm(s->L), S.aux, s->Sp, s->selfDv,
+                         (const u32*)s->cellStart, nl, (const float*)s->lamPrev, (const MlpFwd*)s->wsPacked,
+                         s->wsHb, s->residPart, slotW());
+    } else {
+      XPBI_LAUNCH_W(k_features, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
+                         (const float4*)s->vel[vi], cm(F), cm(s->L), S.aux, s->Sp, s->selfDv,
+                         (const u32*)s->cellStart, nl, (const float*)s->lamPrev, s->wsFeat, s->residPart, slotW());
+    }
     s->prof_mark("features");
     if (loop) hipLaunchKernelGGL(k_resid_c0, dim3(1), dim3(256), 0, st, (const double*)s->residPart, gN, ctl);
     if ((s->wsMode == 1 || s->wsMode == 3) && iterations > 0) {
       // the first Jacobi iteration with Delta-lambda = lambda0 from the MLP (or
       // given per particle) instead of omega d0: its Sp, selfDv and lambda, then
       // solveB (and PBF)
-      if (s->wsMode == 1) {
-        launch_mlp_apply(s);
-      } else {
+      if (s->wsMode == 3) {
         hipLaunchKernelGGL(k_lam0_apply, dim3(cdiv(s->N, 256)), dim3(256), 0, st, (const float*)s->wsFeat,
                            (const u32*)S.id, (const float*)s->lam0ById, s->N, S.aux, s->Sp, s->selfDv);
+        s->prof_mark("lam0");
       }
-      s->prof_mark("mlp");
       XPBI_LAUNCH_W(k_solveB, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
                          (const float4*)s->vel[vi], spc(s->Sp), (const float4*)s->selfDv, (const u32*)s->cellStart,
                          nl, s->vel[vi ^ 1], (const SolveCtl*)nullptr, 0u, water ? s->xc : (float4*)nullptr,

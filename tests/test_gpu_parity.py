@@ -171,6 +171,30 @@ def test_sort_after_motion_and_reorder(oracle):
     sim.close()
 
 
+def test_sort_crowded_cells_in_random_order(oracle):
+    """Cells holding thousands of particles, the array in no cell order at all: the
+    count's slots and the in-cell rank still give numpy's stable sort."""
+    scene = scenes.sand_column(16, 32, 16, 0.01, [0.6, 0.5, 0.6], seed=4, iterations=5)
+    rng = np.random.default_rng(4)
+    n = scene["positions"].shape[0]
+    lo, hi = np.array([0.1, 0.1, 0.1]), np.array(scene["domainMax"], np.float64) - 0.1
+    pos = lo + rng.random((n, 3)) * (hi - lo)
+    clump = rng.random(n) < 0.4  # 40 % in one tiny box, within a single cell
+    pos[clump] = np.array([0.305, 0.205, 0.305]) + rng.random((int(clump.sum()), 3)) * 0.002
+    second = (~clump) & (rng.random(n) < 0.2)  # and 20 % of the rest in another
+    pos[second] = np.array([0.155, 0.405, 0.455]) + rng.random((int(second.sum()), 3)) * 0.002
+    scene["positions"] = pos.astype(np.float32)
+    sim = make_sim(scene)
+    keys = oracle.cell_index(sim.params, scene["positions"])
+    counts = np.bincount(keys, minlength=sim.numCells)
+    print("sort, crowded cells: N", n, "largest cells", np.sort(counts)[-2:], flush=True)
+    assert counts.max() >= 0.4 * n - 200
+    sim.substep_partial(0, False, "SORT")
+    assert np.array_equal(sim.read("PERM"), np.argsort(keys, kind="stable").astype(np.uint32))
+    assert np.array_equal(sim.read("CELLSTART"), (np.cumsum(counts) - counts).astype(np.uint32))
+    sim.close()
+
+
 def test_staged_parity_sand(oracle):
     """computeL, solve, xsph, updateF stages of a landed sand column against the f32 oracle."""
     scene = scenes.sand_column(16, 32, 16, 0.01, [0.6, 0.5, 0.6], seed=3, iterations=20)

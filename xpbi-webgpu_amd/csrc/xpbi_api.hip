@@ -269,11 +269,7 @@ int xpbi_create_batch(const xpbi_scene_desc* descs, int count, xpbi_sim** out) {
   A(dalloc(s, &s->xc, N));
   A(dalloc(s, &s->cellStart, (size_t)C + 1));
   A(dalloc(s, &s->count, C));
-  size_t np = scan_partials_needed(C);
-  size_t nr = radix_hist_needed(N);
-  size_t np2 = scan_partials_needed(nr);
-  A(dalloc(s, &s->scanPart, np > np2 ? np : np2));
-  A(dalloc(s, &s->radixHist, nr > 0 ? nr : 1));
+  A(dalloc(s, &s->scanPart, scan_partials_needed(C)));
   A(dalloc(s, &s->dMats, 8ull * count));
   A(dalloc(s, &s->dKP, (size_t)count));
   A(dalloc(s, &s->vmaxSlot, 2ull * count));
@@ -1008,7 +1004,7 @@ constexpr u32 kNaSlots = 64;  // support-neighbour list slots (more neighbours: 
 constexpr size_t kNaAlign = 256;
 constexpr size_t naUp(size_t b) { return (b + kNaAlign - 1) / kNaAlign * kNaAlign; }
 // the scratch layout: per-particle arrays for kNaMaxN particles, per-cell arrays
-// for kNaMaxC cells, then the scan/radix partials and the scene's tables
+// for kNaMaxC cells, then the scan partials and the scene's tables
 constexpr size_t kNaPerParticle = 2 * (16 + 16 + 16 + 4 + 16 + 4)  // the two particle sets
                                   + 2 * 16                          // velocities
                                   + 2 * 4 + 2 * 4                   // sort keys and values
@@ -1017,7 +1013,7 @@ constexpr size_t kNaPerParticle = 2 * (16 + 16 + 16 + 4 + 16 + 4)  // the two pa
                                   + 2 * kNaSlots + 8                // neighbour list and heads
                                   + 16;                             // predicted positions
 // per-particle and per-cell arrays at the maximum sizes, plus 1 MiB for the
-// scan and radix partials, the tables and the alignment of each array
+// scan partials, the tables and the alignment of each array
 constexpr size_t kNaScratchBytes = kNaPerParticle * (size_t)kNaMaxN + 8ull * kNaMaxC + (1ull << 20);
 
 __device__ __align__(256) unsigned char scratch[kNaScratchBytes];
@@ -1152,9 +1148,7 @@ int xpbi_substep_noalloc(const xpbi_params* p, const xpbi_material* mats, int n,
   s.xc = (float4*)take(16ull * N);
   s.cellStart = (u32*)take(4ull * ((size_t)C + 1));
   s.count = (u32*)take(4ull * C);
-  const size_t nr = radix_hist_needed(N), np = std::max(scan_partials_needed(C), scan_partials_needed(nr));
-  s.radixHist = (u32*)take(4ull * std::max<size_t>(nr, 1));
-  s.scanPart = (u32*)take(4ull * np);
+  s.scanPart = (u32*)take(4ull * scan_partials_needed(C));
   s.dKP = (KP*)take(sizeof(KP));
   s.vmaxSlot = (u32*)take(2 * sizeof(u32));
   s.dMats = (DevMat*)take(8 * sizeof(DevMat));

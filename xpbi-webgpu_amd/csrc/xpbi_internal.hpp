@@ -119,7 +119,6 @@ struct Sim {
   u32* perm = nullptr;        // points into vals[]
   u32* sortedKeys = nullptr;  // points into keys[]
   u32* scanPart = nullptr;
-  u32* radixHist = nullptr;
   // support-neighbour list of the current substep, built by computeL:
   // support-neighbour list: nbrLst[k * N + i] for k < nbrHead[i].y <= nbrK, each
   // a u16 offset from the base nbrHead[i].x (nbrHead[i].y > nbrK: the sweeps use
@@ -187,8 +186,6 @@ void launch_sort(Sim* s);
 void launch_reorder(Sim* s);
 void launch_exclusive_scan(Sim* s, const u32* in, u32 n, u32* out, bool write_total, u32 total);
 size_t scan_partials_needed(u64 n);
-size_t radix_hist_needed(u64 n);
-int radix_passes(u32 C);
 void launch_solve_stages(Sim* s, int iterations, bool water, int stop);
 void launch_copy(Sim* s, const float4* src, float4* dst, u64 n);
 void mlp_pack_host(const float* params, void* packed);

@@ -9,10 +9,12 @@
 //  * exclusive scan of the counts -> cellStart[0..C] with cellStart[C] = N; the
 //    scan is a wave64 shuffle scan inside reduce-then-scan blocks, replacing the
 //    reference's O(NB^2) serial block-sum loop (simkernels.js:117-121).
-//  * the permutation is a stable LSD radix sort of (cell id, index) with 8-bit
-//    digits; each block ranks its tile with wave64 ballot match masks, so the
-//    order inside a cell is ascending original index -- deterministic, unlike
-//    the reference's atomicSub scatter (simkernels.js:138).
+//  * the permutation is the stable counting sort: ordered by cell, and inside a
+//    cell by ascending original index -- deterministic, unlike the reference's
+//    atomicSub scatter (simkernels.js:138).  The count's atomics hand each
+//    particle a slot in its cell (in whatever order they land); k_cell_scatter
+//    puts the particle there, and k_cell_rank moves it to its stable place,
+//    cellStart + the number of the cell's particles with a smaller index.
 //  * k_reorder gathers every particle array into cell order in one pass and
 //    fuses the per-particle work of scatter: V^n = V0 clamp(det F, 0.1, 10),
 //    v += dt g, lambda = 0 (simkernels.js:142-146).
@@ -23,19 +25,18 @@ namespace xpbi {
 constexpr int SCAN_T = 256;
 constexpr int SCAN_ITEMS = 16;
 constexpr int SCAN_TILE = SCAN_T * SCAN_ITEMS;  // 4096 elements per block
-constexpr int RADIX_T = 256;
-constexpr int RADIX_ROUNDS = 8;
-constexpr int RADIX_TILE = RADIX_T * RADIX_ROUNDS;  // 2048 keys per block
 
 __global__ __launch_bounds__(256) void k_fill_u32(u32* __restrict__ p, u32 n, u32 v) {
   u32 i = blockIdx.x * 256u + threadIdx.x;
   if (i < n) p[i] = v;
 }
 
-// countParticles (simkernels.js:79-84) + key output for the radix sort
+// countParticles (simkernels.js:79-84): the cell id of each particle and the
+// per-cell counts, and each particle's slot among its cell's particles in the
+// order the atomics landed
 template <bool kPersist>
 __global__ __launch_bounds__(256) void k_hash_count(const KP* __restrict__ kps, WorkTab W, const float4* __restrict__ pos, u32* __restrict__ keys,
-                                                    u32* __restrict__ count) {
+                                                    u32* __restrict__ count, u32* __restrict__ slot) {
   forWork<kPersist>(W, [&](u32 scene, u32 first) {
     const KP& P = kps[scene];
     const u32 i = first + threadIdx.x;
@@ -54,12 +55,49 @@ __global__ __launch_bounds__(256) void k_hash_count(const KP* __restrict__ kps, 
     const u32 prev = __shfl_up(c, 1u, 64);
     const bool head = lane == 0u || prev != c;
     const uint64_t heads = __ballot(head);
+    const uint64_t upto = (2ull << lane) - 1ull;  // lanes 0..lane (lane 63: all)
+    u32 old = 0u;
     if (valid && head) {
-      const uint64_t after = heads & ~((2ull << lane) - 1ull);  // heads of the lanes above (lane 63: none)
+      const uint64_t after = heads & ~upto;  // heads of the lanes above
       const u32 next = after ? (u32)__ffsll((unsigned long long)after) - 1u : 64u;
-      atomicAdd(&count[c], next - lane);
+      old = atomicAdd(&count[c], next - lane);
     }
+    // the run's slots follow the count its head's atomic found
+    const u32 hl = 63u - (u32)__clzll((long long)(heads & upto));
+    const u32 base = __shfl(old, hl, 64);
+    if (valid) slot[i] = base + (lane - hl);
   });
+}
+
+// each particle into its slot of its cell (cellStart[c] + slot): cell order,
+// the order inside a cell that of the atomics
+__global__ __launch_bounds__(256) void k_cell_scatter(const u32* __restrict__ keys, const u32* __restrict__ slot,
+                                                      const u32* __restrict__ cellStart, u32 n,
+                                                      u32* __restrict__ ckeys, u32* __restrict__ cidx) {
+  const u32 i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n) return;
+  const u32 c = keys[i];
+  const u32 d = cellStart[c] + slot[i];
+  ckeys[d] = c;
+  cidx[d] = i;
+}
+
+// the stable place of the particle in slot s: its cell's start plus the number
+// of the cell's particles with a smaller original index.  The lanes of a wave
+// mostly share a cell and walk the same entries; the work is the sum over cells
+// of their squared counts.
+__global__ __launch_bounds__(256) void k_cell_rank(const u32* __restrict__ ckeys, const u32* __restrict__ cidx,
+                                                   const u32* __restrict__ cellStart, u32 n,
+                                                   u32* __restrict__ skeys, u32* __restrict__ perm) {
+  const u32 s = blockIdx.x * 256u + threadIdx.x;
+  if (s >= n) return;
+  const u32 c = ckeys[s], i = cidx[s];
+  const u32 b = cellStart[c], e = cellStart[c + 1];
+  u32 r = 0;
+#pragma unroll 4
+  for (u32 k = b; k < e; k++) r += cidx[k] < i ? 1u : 0u;
+  skeys[b + r] = c;
+  perm[b + r] = i;
 }
 
 // ---------------------------------------------------------------- scan
@@ -138,67 +176,6 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_down(const u32* in, u32 n, cons
   if (wtot && blockIdx.x == gridDim.x - 1 && threadIdx.x == SCAN_T - 1) out[n] = total_if_known;
 }
 
-// ---------------------------------------------------------------- radix sort
-// digit histogram of one 2048-key tile -> bh[digit * nblk + blk]
-__global__ __launch_bounds__(RADIX_T) void k_radix_hist(const u32* __restrict__ keys, u32 n, u32 shift,
-                                                       u32* __restrict__ bh, u32 nblk) {
-  __shared__ u32 h[256];
-  h[threadIdx.x] = 0;
-  __syncthreads();
-  u32 base = blockIdx.x * (u32)RADIX_TILE;
-#pragma unroll
-  for (int r = 0; r < RADIX_ROUNDS; r++) {
-    u32 idx = base + r * RADIX_T + threadIdx.x;
-    if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
-  }
-  __syncthreads();
-  bh[threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
-}
-
-// stable scatter of one tile; off[digit * nblk + blk] = global start of that
-// digit's keys from this block.  vals == nullptr means vals[i] = i.
-__global__ __launch_bounds__(RADIX_T) void k_radix_scatter(const u32* __restrict__ kin, const u32* __restrict__ vin,
-                                                          u32* __restrict__ kout, u32* __restrict__ vout, u32 n,
-                                                          u32 shift, const u32* __restrict__ off, u32 nblk) {
-  __shared__ u32 s_run[256];
-  __shared__ u32 s_base[256];
-  __shared__ u32 s_wcnt[4][256];
-  const u32 tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
-  s_run[tid] = 0;
-  s_base[tid] = off[tid * nblk + blockIdx.x];
-  const uint64_t ltmask = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
-  u32 base = blockIdx.x * (u32)RADIX_TILE;
-  for (int r = 0; r < RADIX_ROUNDS; r++) {
-#pragma unroll
-    for (int k = 0; k < 4; k++) s_wcnt[k][tid] = 0;
-    __syncthreads();
-    u32 idx = base + r * RADIX_T + tid;
-    bool valid = idx < n;
-    u32 key = valid ? kin[idx] : 0u;
-    u32 val = valid ? (vin ? vin[idx] : idx) : 0u;
-    u32 d = (key >> shift) & 255u;
-    uint64_t m = __ballot(valid);
-#pragma unroll
-    for (int bit = 0; bit < 8; bit++) {
-      uint64_t bb = __ballot((d >> bit) & 1u);
-      m &= ((d >> bit) & 1u) ? bb : ~bb;
-    }
-    u32 rank = (u32)__popcll(m & ltmask);
-    if (valid && rank == 0) s_wcnt[w][d] = (u32)__popcll(m);
-    __syncthreads();
-    if (valid) {
-      u32 pre = s_run[d];
-      for (u32 k = 0; k < w; k++) pre += s_wcnt[k][d];
-      u32 dst = s_base[d] + pre + rank;
-      kout[dst] = key;
-      vout[dst] = val;
-    }
-    __syncthreads();
-    s_run[tid] += s_wcnt[0][tid] + s_wcnt[1][tid] + s_wcnt[2][tid] + s_wcnt[3][tid];
-    __syncthreads();
-  }
-}
-
 // ---------------------------------------------------------------- reorder
 template <bool kPersist>
 __global__ __launch_bounds__(256) void k_reorder(const KP* __restrict__ kps, WorkTab W, const u32* __restrict__ perm, PSetC src,
@@ -250,42 +227,25 @@ void launch_exclusive_scan(Sim* s, const u32* in, u32 n, u32* out, bool write_to
 }
 
 size_t scan_partials_needed(u64 n) { return (size_t)cdiv(n, SCAN_TILE) + 1; }
-size_t radix_hist_needed(u64 n) { return (size_t)256 * cdiv(n, RADIX_TILE); }
-
-int radix_passes(u32 C) {
-  u32 maxKey = C > 0 ? C - 1 : 0;
-  int bits = 0;
-  while (bits < 32 && (maxKey >> bits) != 0) bits++;
-  int p = (bits + 7) / 8;
-  return p < 1 ? 1 : p;
-}
 
 void launch_sort(Sim* s) {
   const u32 N = s->N, C = s->C;
   hipLaunchKernelGGL(k_fill_u32, dim3(cdiv(C, 256)), dim3(256), 0, s->stream, s->count, C, 0u);
   s->prof_mark("clear");
   XPBI_LAUNCH_W(k_hash_count, dim3(s->grid), dim3(256), 0, s->stream, (const KP*)s->dKP, s->work(),
-                     (const float4*)s->set[s->cur].pos, s->keys[0], s->count);
+                     (const float4*)s->set[s->cur].pos, s->keys[0], s->count, s->vals[0]);
   s->prof_mark("countParticles");
   launch_exclusive_scan(s, s->count, C, s->cellStart, true, N);
   s->prof_mark("scanCells");
-  const u32 nblk = cdiv(N, RADIX_TILE);
-  const int passes = radix_passes(C);
-  const u32* vin = nullptr;
-  int kb = 0;
-  for (int p = 0; p < passes; p++) {
-    u32 shift = 8u * (u32)p;
-    hipLaunchKernelGGL(k_radix_hist, dim3(nblk), dim3(RADIX_T), 0, s->stream, (const u32*)s->keys[kb], N, shift,
-                       s->radixHist, nblk);
-    launch_exclusive_scan(s, s->radixHist, 256u * nblk, s->radixHist, false, 0u);
-    hipLaunchKernelGGL(k_radix_scatter, dim3(nblk), dim3(RADIX_T), 0, s->stream, (const u32*)s->keys[kb], vin,
-                       s->keys[kb ^ 1], s->vals[kb ^ 1], N, shift, (const u32*)s->radixHist, nblk);
-    vin = s->vals[kb ^ 1];
-    kb ^= 1;
+  if (N > 0) {
+    hipLaunchKernelGGL(k_cell_scatter, dim3(cdiv(N, 256)), dim3(256), 0, s->stream, (const u32*)s->keys[0],
+                       (const u32*)s->vals[0], (const u32*)s->cellStart, N, s->keys[1], s->vals[1]);
+    hipLaunchKernelGGL(k_cell_rank, dim3(cdiv(N, 256)), dim3(256), 0, s->stream, (const u32*)s->keys[1],
+                       (const u32*)s->vals[1], (const u32*)s->cellStart, N, s->keys[0], s->vals[0]);
   }
-  s->perm = s->vals[kb];
-  s->sortedKeys = s->keys[kb];
-  s->prof_mark("radixSort");
+  s->perm = s->vals[0];
+  s->sortedKeys = s->keys[0];
+  s->prof_mark("cellSort");
 }
 
 void launch_reorder(Sim* s) {

@@ -26,9 +26,14 @@ constexpr int SCAN_T = 256;
 constexpr int SCAN_ITEMS = 16;
 constexpr int SCAN_TILE = SCAN_T * SCAN_ITEMS;  // 4096 elements per block
 
+// p[0..n) = v, four items per thread in one 16-B store (p is 16-B aligned)
 __global__ __launch_bounds__(256) void k_fill_u32(u32* __restrict__ p, u32 n, u32 v) {
-  u32 i = blockIdx.x * 256u + threadIdx.x;
-  if (i < n) p[i] = v;
+  const u32 q = blockIdx.x * 256u + threadIdx.x;
+  if (4u * q + 4u <= n) {
+    ((uint4*)p)[q] = make_uint4(v, v, v, v);
+  } else {
+    for (u32 i = 4u * q; i < n; i++) p[i] = v;
+  }
 }
 
 // countParticles (simkernels.js:79-84): the cell id of each particle and the
@@ -129,13 +134,20 @@ __device__ __forceinline__ u32 block_excl_scan(u32 v, u32* total) {
   return off + inc - v;
 }
 
+// the tile's sum: loads of 16 B each, consecutive lanes on consecutive
+// addresses (the sum does not need each thread's items to be contiguous)
 __global__ __launch_bounds__(SCAN_T) void k_scan_reduce(const u32* __restrict__ in, u32 n, u32* __restrict__ part) {
-  u32 base = blockIdx.x * (u32)SCAN_TILE + threadIdx.x * (u32)SCAN_ITEMS;
+  const u32 tile = blockIdx.x * (u32)SCAN_TILE;
   u32 s = 0;
+  if (tile + (u32)SCAN_TILE <= n) {
+    const uint4* p = (const uint4*)(in + tile);
 #pragma unroll
-  for (int k = 0; k < SCAN_ITEMS; k++) {
-    u32 idx = base + k;
-    if (idx < n) s += in[idx];
+    for (int k = 0; k < SCAN_ITEMS / 4; k++) {
+      const uint4 q = p[k * SCAN_T + threadIdx.x];
+      s += q.x + q.y + q.z + q.w;
+    }
+  } else {
+    for (u32 idx = tile + threadIdx.x; idx < n; idx += SCAN_T) s += in[idx];
   }
   u32 tot;
   (void)block_excl_scan(s, &tot);
@@ -153,25 +165,46 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_partials(u32* __restrict__ part
     carry += tot;
   }
 }
-// out[i] = exclusive prefix of in (in may alias out); out[n] = total if wtot
+// out[i] = exclusive prefix of in (in may alias out); out[n] = total if wtot.
+// Each thread scans its 16 consecutive items, moved as four 16-B loads and
+// stores (in and out are 16-B aligned).
 __global__ __launch_bounds__(SCAN_T) void k_scan_down(const u32* in, u32 n, const u32* __restrict__ part, u32* out,
                                                      int wtot, u32 total_if_known) {
-  u32 base = blockIdx.x * (u32)SCAN_TILE + threadIdx.x * (u32)SCAN_ITEMS;
+  const u32 base = blockIdx.x * (u32)SCAN_TILE + threadIdx.x * (u32)SCAN_ITEMS;
+  const bool full = base + (u32)SCAN_ITEMS <= n;
   u32 v[SCAN_ITEMS];
+  if (full) {
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS / 4; k++) {
+      const uint4 q = ((const uint4*)(in + base))[k];
+      v[4 * k] = q.x;
+      v[4 * k + 1] = q.y;
+      v[4 * k + 2] = q.z;
+      v[4 * k + 3] = q.w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++) v[k] = base + k < n ? in[base + k] : 0u;
+  }
   u32 s = 0;
 #pragma unroll
-  for (int k = 0; k < SCAN_ITEMS; k++) {
-    u32 idx = base + k;
-    v[k] = idx < n ? in[idx] : 0u;
-    s += v[k];
-  }
+  for (int k = 0; k < SCAN_ITEMS; k++) s += v[k];
   u32 tot;
   u32 run = part[blockIdx.x] + block_excl_scan(s, &tot);
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; k++) {
-    u32 idx = base + k;
-    if (idx < n) out[idx] = run;
-    run += v[k];
+    const u32 x = v[k];
+    v[k] = run;
+    run += x;
+  }
+  if (full) {
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS / 4; k++)
+      ((uint4*)(out + base))[k] = make_uint4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; k++)
+      if (base + k < n) out[base + k] = v[k];
   }
   if (wtot && blockIdx.x == gridDim.x - 1 && threadIdx.x == SCAN_T - 1) out[n] = total_if_known;
 }
@@ -217,6 +250,7 @@ __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, fl
 // ---------------------------------------------------------------- host side
 static inline u32 cdiv(u64 a, u64 b) { return (u32)((a + b - 1) / b); }
 
+// in and out 16-B aligned (device allocations and the scratch carve are 256-B aligned)
 void launch_exclusive_scan(Sim* s, const u32* in, u32 n, u32* out, bool write_total, u32 total) {
   u32 nb = cdiv(n, SCAN_TILE);
   if (nb == 0) nb = 1;
@@ -230,7 +264,7 @@ size_t scan_partials_needed(u64 n) { return (size_t)cdiv(n, SCAN_TILE) + 1; }
 
 void launch_sort(Sim* s) {
   const u32 N = s->N, C = s->C;
-  hipLaunchKernelGGL(k_fill_u32, dim3(cdiv(C, 256)), dim3(256), 0, s->stream, s->count, C, 0u);
+  hipLaunchKernelGGL(k_fill_u32, dim3(cdiv(cdiv(C, 4), 256)), dim3(256), 0, s->stream, s->count, C, 0u);
   s->prof_mark("clear");
   XPBI_LAUNCH_W(k_hash_count, dim3(s->grid), dim3(256), 0, s->stream, (const KP*)s->dKP, s->work(),
                      (const float4*)s->set[s->cur].pos, s->keys[0], s->count, s->vals[0]);

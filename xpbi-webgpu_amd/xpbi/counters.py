@@ -24,7 +24,9 @@ prints per kernel the mean over its last L dispatches of:
 The second form summarises a `rocprofv3 --kernel-trace` database instead: per
 kernel the calls, mean, min and max duration and share of the summed kernel
 time, over the whole run and over the last K substeps (counted back from the
-end by k_integrate, the last kernel of a substep).
+end by k_integrate, the last kernel of a substep).  Launches shorter than a fifth
+of their kernel's median get a "(short)" line of their own: the solve launches
+queued past the end of a residual-driven loop, which return at entry.
 
 Passes of a few substeps profile cleanly; passes over hundreds of substeps have
 ended in a segmentation fault inside the profiler's dispatch interception, so a
@@ -195,6 +197,18 @@ def trace_table(db, last_substeps=0):
             a.setdefault(n, []).append((t1 - t0) / 1e3)
         return a
     every, window = agg(0), agg(start)
+    # launches far shorter than their kernel's median (a solve launch after the
+    # residual test has stopped the loop returns at entry) get a line of their own
+    for k in list(window):
+        v = sorted(window[k])
+        med = v[len(v) // 2]
+        short = [x for x in window[k] if x < 0.2 * med]
+        if short:
+            window[k] = [x for x in window[k] if x >= 0.2 * med]
+            window[k + " (short)"] = short
+            e = every[k]
+            every[k + " (short)"] = [x for x in e if x < 0.2 * med]
+            every[k] = [x for x in e if x >= 0.2 * med]
     tot = sum(sum(v) for v in window.values())
     wall = (rows[-1][1] - rows[start][0]) / 1e3 if rows else 0.0
     out = [f"dispatches: {len(rows)}; window: {len(rows) - start} dispatches from {start}"

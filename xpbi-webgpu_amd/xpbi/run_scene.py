@@ -2,6 +2,7 @@
 
     python -m xpbi.run_scene WORKLOAD [--substeps K] [--load PATH] [--save PATH]
                              [--s-corr K] [--every N] [--copy-mib M]
+                             [--tol R [--cap I]] [--warmstart NET.npy]
 
 WORKLOAD is one of the benchmark workloads (sand_column_1m, sand_column_128k,
 dam_breach_1m, vm_poke_1m) or the name of a reference scene ("Sand: column
@@ -12,7 +13,10 @@ velocity sits at pos + vel * t, t = substep count * dt, set before each substep.
 Prints one progress line every N substeps (default 50) and a final line with the
 substep count and the wall time per substep.  --copy-mib M ends the run with one
 M MiB device copy through the library's copy kernel: the calibration of the byte
-counters in xpbi.counters.
+counters in xpbi.counters.  --tol R runs each substep's Jacobi loop to residual R
+(at most --cap iterations, default 200) instead of the workload's fixed count;
+--warmstart starts it from the learned warm start of a saved network
+(python -m xpbi.warmstart --out NET.npy).
 
 A profiler run of a few substeps of a late state of a scene is then
     python -m xpbi.run_scene dam_breach_1m --substeps 100 --save state.npz
@@ -42,7 +46,8 @@ def make_scene(name):
     raise KeyError(f"unknown workload {name!r}; known: {sorted(WORKLOADS) + sorted(scenes.SCENES)}")
 
 
-def run(name, substeps, load=None, save=None, s_corr=None, every=50, out=sys.stdout, copy_mib=0):
+def run(name, substeps, load=None, save=None, s_corr=None, every=50, out=sys.stdout, copy_mib=0, tol=None,
+        cap=200, warmstart=None):
     """name: a workload or reference scene name, or a scene dict."""
     sc = make_scene(name) if isinstance(name, str) else dict(name)
     name = name if isinstance(name, str) else sc.get("info", "scene")
@@ -54,6 +59,12 @@ def run(name, substeps, load=None, save=None, s_corr=None, every=50, out=sys.std
     col = sc.get("collider")
     k0 = checkpoint.load(sim, load) if load else 0
     sim.writeParams(params_from_scene(sc, dt=dt))
+    if warmstart:
+        from . import warmstart as ws
+        sim.set_warmstart(ws.load(warmstart))
+    if tol is not None:
+        sim.set_tolerance(tol, 1)
+        I = cap
     t0 = time.perf_counter()
     for k in range(k0, k0 + substeps):
         if col is not None:
@@ -84,8 +95,12 @@ def main(argv=None):
     ap.add_argument("--s-corr", type=float, default=None)
     ap.add_argument("--every", type=int, default=50)
     ap.add_argument("--copy-mib", type=int, default=0)
+    ap.add_argument("--tol", type=float, default=None)
+    ap.add_argument("--cap", type=int, default=200)
+    ap.add_argument("--warmstart")
     a = ap.parse_args(argv)
-    run(a.workload, a.substeps, a.load, a.save, a.s_corr, a.every, copy_mib=a.copy_mib)
+    run(a.workload, a.substeps, a.load, a.save, a.s_corr, a.every, copy_mib=a.copy_mib, tol=a.tol, cap=a.cap,
+        warmstart=a.warmstart)
 
 
 if __name__ == "__main__":

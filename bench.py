@@ -448,6 +448,8 @@ def main():
     ap.add_argument("--no-configs", action="store_true", help="skip the mixed (c3), poke (c5) and 2D (c1) scenes")
     ap.add_argument("--only-2d", action="store_true", help="run only the 2D 40x80 fp64 column; print its object")
     ap.add_argument("--no-counters", action="store_true", help="skip the rocprofv3 counter passes")
+    ap.add_argument("--eager", action="store_true",
+                    help="time the fixed-iteration substeps as eager launches (no autotune, no HIP graph)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: a simulation runs on one device and the engine has no "
@@ -473,14 +475,18 @@ def main():
         from xpbi import checkpoint
         ckpt = os.path.join(tempfile.mkdtemp(prefix="xpbi_bench_"), "settled.npz")
         checkpoint.save(sim, ckpt, args.settle)
-    blocks, tune_ms = sim.autotune(I, False, substeps=2)
-    print(f"[bench] autotuned launch: {blocks} blocks (0 = one per run), {tune_ms:.3f} ms/substep",
-          file=sys.stderr, flush=True)
-    sim.graph_step(args.warmup, I, False)
+    if args.eager:
+        blocks, advance = 0, sim.step
+    else:
+        blocks, tune_ms = sim.autotune(I, False, substeps=2)
+        print(f"[bench] autotuned launch: {blocks} blocks (0 = one per run), {tune_ms:.3f} ms/substep",
+              file=sys.stderr, flush=True)
+        advance = sim.graph_step
+    advance(args.warmup, I, False)
     sim.sync()
     print("[bench] timed region", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    sim.graph_step(args.steps, I, False)
+    advance(args.steps, I, False)
     sim.sync()
     ms_per_step = (time.perf_counter() - t0) * 1e3 / args.steps
     value = N / (ms_per_step / 1e3)
@@ -513,8 +519,10 @@ def main():
     gather = dict(achieved_GBs=round(g_bytes / (g_ms / 1e3) / 1e9, 1),
                   frac=round(g_bytes / (g_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4))
 
+    fixed_timing = ("eager substeps, one block per 256-particle run" if args.eager
+                    else "HIP-graph replay of two captured substeps, autotuned launch shape")
     fixed = dict(value=round(value, 1), ms_per_step=round(ms_per_step, 4), steps=args.steps, iterations=I,
-                 timing="HIP-graph replay of two captured substeps, autotuned launch shape", launch_blocks=blocks)
+                 timing=fixed_timing, launch_blocks=blocks)
     resid, net, train = None, None, None
     if not args.no_residual:
         print("[bench] Jacobi to residual 1e-4: plain vs learned warm start", file=sys.stderr, flush=True)
@@ -588,8 +596,7 @@ def main():
                               "host reads every iteration from pinned memory without stalling the stream, staying at most 2 "
                               "iterations ahead; no host read between "
                               "timed substeps (the iteration counts come from an identical untimed pass); plain "
-                              "and learned from the same snapshot" if resid else
-                              "HIP-graph replay of two captured substeps, autotuned launch shape"),
+                              "and learned from the same snapshot" if resid else fixed_timing),
                    "launch_blocks": blocks},
         "plain_value": resid["plain_particle_substeps_per_s"] if resid else None,
         "learned_vs_plain_time_ratio": resid["time_ratio"] if resid else None,

@@ -45,10 +45,19 @@ __global__ __launch_bounds__(256) void k_computeL(const KP* __restrict__ kps, Wo
                                                   const float4* __restrict__ pos, const float4* __restrict__ vin,
                                                   const u32* __restrict__ cs, M3P L, u16* __restrict__ nLst,
                                                   uint2* __restrict__ nHead, u32 K, u32 NT,
-                                                  u32* __restrict__ vmaxSlot, u32 nSlots) {
-  // both velocity-bound slots of every scene start the solve cleared
-  if (blockIdx.x == 0)
+                                                  u32* __restrict__ vmaxSlot, u32 nSlots, SolveCtl* __restrict__ ctl) {
+  // both velocity-bound slots of every scene start the solve cleared, and so does
+  // the residual loop's control record (ctl: a loop follows)
+  if (blockIdx.x == 0) {
     for (u32 k = threadIdx.x; k < nSlots; k += 256u) vmaxSlot[k] = 0u;
+    if (ctl && threadIdx.x == 0) {
+      ctl->done = 0u;
+      ctl->iters = 0u;
+      ctl->c0preset = 0u;
+      ctl->c0sq = 0.0;
+      ctl->r2 = 0.0;
+    }
+  }
   // each thread's first kStage list entries, written out slot by slot at the
   // end so that a store covers consecutive particles (entries found during the
   // walk land in different slots per lane)
@@ -314,16 +323,6 @@ __device__ __forceinline__ void residTest(const ResidTest& t, u32 k) {
     }
   }
   __syncthreads();  // s_out is read above before a later call may write it
-}
-
-__global__ void k_ctl_reset(SolveCtl* __restrict__ ctl) {
-  if (threadIdx.x == 0) {
-    ctl->done = 0u;
-    ctl->iters = 0u;
-    ctl->c0preset = 0u;
-    ctl->c0sq = 0.0;
-    ctl->r2 = 0.0;
-  }
 }
 
 // after a loop that stopped early the velocity ping-pong has moved `iters`
@@ -856,10 +855,12 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
   hipStream_t st = s->stream;
   int vi = s->vcur;
   const u32 nScenes = (u32)s->sc.size();
+  const bool loop = s->tol > 0.0f && stop != XPBI_STAGE_COMPUTE_L;
+  SolveCtl* ctl = loop ? s->ctl : nullptr;
   XPBI_LAUNCH_W(k_computeL, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
                      (const float4*)s->vel[vi],
                      (const u32*)s->cellStart, s->L, s->nbrLst, s->nbrHead, s->nbrK, s->N, s->vmaxSlot,
-                     2u * nScenes);
+                     2u * nScenes, ctl);
   const NLc nl{s->nbrHead, s->nbrLst, s->nbrK, s->N};
   // the velocity-bound slot the next solveA raises and its solveB reads
   s->vslot = 0;
@@ -868,10 +869,7 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
   s->prof_mark("computeL");
   if (stop == XPBI_STAGE_COMPUTE_L) return;
   s->residIters = 0;
-  const bool loop = s->tol > 0.0f;
-  SolveCtl* ctl = loop ? s->ctl : nullptr;
   const double tol2 = (double)s->tol * (double)s->tol;
-  if (loop) hipLaunchKernelGGL(k_ctl_reset, dim3(1), dim3(64), 0, st, ctl);
   const u32 gen = loop ? ++s->solveGen : 0u;
   const float* hbHint = s->wsMode == 1 ? (const float*)s->wsHb : nullptr;
   int warm = 0;  // the learned first iteration ran

@@ -291,6 +291,9 @@ struct ResidTest {
   double tol2;
   CtlMirror* mirror;
   u32 gen;
+  // the solveB of the learned first iteration: part holds the features pass's
+  // partials, whose sum C^2 becomes the residual's denominator; no test
+  u32 c0only;
 };
 __device__ __forceinline__ void residTest(const ResidTest& t, u32 k) {
   __shared__ double s_out[2];
@@ -301,7 +304,12 @@ __device__ __forceinline__ void residTest(const ResidTest& t, u32 k) {
   }
   blockReduce2(a, b, s_out);
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (t.c0only) {
+    if (threadIdx.x == 0) {
+      t.ctl->c0sq = s_out[1];
+      t.ctl->c0preset = 1u;
+    }
+  } else if (threadIdx.x == 0) {
     if (t.out) {
       t.out[0] = s_out[0];
       t.out[1] = s_out[1];
@@ -887,8 +895,12 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
                          (const u32*)s->cellStart, nl, (const float*)s->lamPrev, s->wsFeat, s->residPart, slotW());
     }
     s->prof_mark("features");
-    if (loop) hipLaunchKernelGGL(k_resid_c0, dim3(1), dim3(256), 0, st, (const double*)s->residPart, gN, ctl);
-    if ((s->wsMode == 1 || s->wsMode == 3) && iterations > 0) {
+    const bool learnedFirst = (s->wsMode == 1 || s->wsMode == 3) && iterations > 0;
+    // sum C^2 of the features pass: in the first block of the learned first
+    // iteration's solveB, or here when there is none
+    if (loop && !learnedFirst)
+      hipLaunchKernelGGL(k_resid_c0, dim3(1), dim3(256), 0, st, (const double*)s->residPart, gN, ctl);
+    if (learnedFirst) {
       // the first Jacobi iteration with Delta-lambda = lambda0 from the MLP (or
       // given per particle) instead of omega d0: its Sp, selfDv and lambda, then
       // solveB (and PBF)
@@ -900,7 +912,9 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
       XPBI_LAUNCH_W(k_solveB, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,
                          (const float4*)s->vel[vi], spc(s->Sp), (const float4*)s->selfDv, (const u32*)s->cellStart,
                          nl, s->vel[vi ^ 1], (const SolveCtl*)nullptr, 0u, water ? s->xc : (float4*)nullptr,
-                         ResidTest{}, (const u32*)slotW(), slotNext(), nScenes);
+                         loop ? ResidTest{(const double*)s->residPart, gN, nullptr, ctl, 0.0, nullptr, 0u, 1u}
+                              : ResidTest{},
+                         (const u32*)slotW(), slotNext(), nScenes);
       s->vslot ^= 1;
       s->prof_mark("solveB");
       vi ^= 1;

@@ -167,9 +167,11 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_partials(u32* __restrict__ part
 }
 // out[i] = exclusive prefix of in (in may alias out); out[n] = total if wtot.
 // Each thread scans its 16 consecutive items, moved as four 16-B loads and
-// stores (in and out are 16-B aligned).
+// stores (in and out are 16-B aligned).  The block's offset is part[blockIdx.x]
+// after k_scan_partials, or with sumPart the sum of the tile totals before it
+// (the raw part of k_scan_reduce; integer sums, the same in any order).
 __global__ __launch_bounds__(SCAN_T) void k_scan_down(const u32* in, u32 n, const u32* __restrict__ part, u32* out,
-                                                     int wtot, u32 total_if_known) {
+                                                     int wtot, u32 total_if_known, int sumPart) {
   const u32 base = blockIdx.x * (u32)SCAN_TILE + threadIdx.x * (u32)SCAN_ITEMS;
   const bool full = base + (u32)SCAN_ITEMS <= n;
   u32 v[SCAN_ITEMS];
@@ -189,8 +191,16 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_down(const u32* in, u32 n, cons
   u32 s = 0;
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; k++) s += v[k];
+  u32 off;
+  if (sumPart) {
+    u32 acc = 0;
+    for (u32 j = threadIdx.x; j < blockIdx.x; j += SCAN_T) acc += part[j];
+    (void)block_excl_scan(acc, &off);
+  } else {
+    off = part[blockIdx.x];
+  }
   u32 tot;
-  u32 run = part[blockIdx.x] + block_excl_scan(s, &tot);
+  u32 run = off + block_excl_scan(s, &tot);
 #pragma unroll
   for (int k = 0; k < SCAN_ITEMS; k++) {
     const u32 x = v[k];
@@ -254,10 +264,14 @@ static inline u32 cdiv(u64 a, u64 b) { return (u32)((a + b - 1) / b); }
 void launch_exclusive_scan(Sim* s, const u32* in, u32 n, u32* out, bool write_total, u32 total) {
   u32 nb = cdiv(n, SCAN_TILE);
   if (nb == 0) nb = 1;
+  // up to kSumPart tiles each down-sweep block sums the totals before it (at
+  // most kSumPart reads per block); beyond, a single block scans them first
+  constexpr u32 kSumPart = 2048;
+  const int sumPart = nb <= kSumPart ? 1 : 0;
   hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_T), 0, s->stream, in, n, s->scanPart);
-  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(SCAN_T), 0, s->stream, s->scanPart, nb);
+  if (!sumPart) hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(SCAN_T), 0, s->stream, s->scanPart, nb);
   hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_T), 0, s->stream, in, n, (const u32*)s->scanPart, out,
-                     write_total ? 1 : 0, total);
+                     write_total ? 1 : 0, total, sumPart);
 }
 
 size_t scan_partials_needed(u64 n) { return (size_t)cdiv(n, SCAN_TILE) + 1; }

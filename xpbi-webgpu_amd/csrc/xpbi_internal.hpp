@@ -50,9 +50,11 @@ struct SolveCtl {
 // device straight into pinned host memory, so that the host polls it while the
 // stream keeps running (no copy, no synchronisation)
 struct CtlMirror {
-  u32 gen;    // solve the fields below belong to (written last)
+  // iters and done form one 8-byte word, written and read whole, so that the
+  // host never pairs a new iteration count with the flag of an earlier test
   u32 iters;  // iterations whose residual test has run
   u32 done;   // converged
+  u32 gen;    // solve the fields above belong to (written last)
   u32 pad;
 };
 

@@ -323,8 +323,7 @@ __device__ __forceinline__ void residTest(const ResidTest& t, u32 k) {
       *reinterpret_cast<uint2*>(ctl) = make_uint2(done, k + 1u);
       if (t.mirror) {
         volatile CtlMirror* m = t.mirror;
-        m->iters = k + 1;
-        m->done = done;
+        *reinterpret_cast<volatile uint64_t*>(m) = (uint64_t)(k + 1u) | ((uint64_t)done << 32);
         __threadfence_system();
         m->gen = t.gen;
       }
@@ -836,8 +835,9 @@ static bool polledDone(Sim* s, u32 gen, int need) {
   for (u32 spin = 0;; spin++) {
     if (m->gen == gen) {
       std::atomic_thread_fence(std::memory_order_acquire);
-      if (m->done) return true;
-      if ((int)m->iters > need) return false;
+      const uint64_t w = *reinterpret_cast<volatile const uint64_t*>(m);
+      if ((u32)(w >> 32)) return true;  // done
+      if ((int)(u32)w > need) return false;
     } else if (need < 0) {
       return false;
     }

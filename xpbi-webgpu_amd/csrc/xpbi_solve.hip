@@ -862,7 +862,9 @@ void launch_solve_stages(Sim* s, int iterations, bool water, int stop) {
   M3P F{S.Fa, S.Fb, S.Fc};
   hipStream_t st = s->stream;
   int vi = s->vcur;
-  const u32 nScenes = (u32)s->sc.size();
+  // velocity-bound slots per scene (two sets of nScenes): the allocation-free
+  // substep's host record runs one scene and keeps no scene list
+  const u32 nScenes = s->sc.empty() ? 1u : (u32)s->sc.size();
   const bool loop = s->tol > 0.0f && stop != XPBI_STAGE_COMPUTE_L;
   SolveCtl* ctl = loop ? s->ctl : nullptr;
   XPBI_LAUNCH_W(k_computeL, dim3(gN), dim3(256), 0, st, P, W, (const DevMat*)s->dMats, (const float4*)S.pos,

@@ -6,7 +6,8 @@
   Jacobi bit for bit;
 * a learned substep (fixed iteration count, so that both stop together) matches the
   oracle's learned substep to the bar of the plain parity tests, on DP sand and on
-  Herschel-Bulkley goo (whose return map takes the network's bracket hint).
+  Herschel-Bulkley goo (whose return map takes the network's bracket hint), and at
+  full size on the benchmark's 1M DP column.
 """
 import numpy as np
 import pytest
@@ -134,6 +135,32 @@ def test_learned_substep_matches_oracle(oracle, name):
         oracle.substep(st, sim.params, mats, I, water, warm=net)
     assert np.array_equal(sim.read("PERM"), st32.sortedIdx)
     compare(sim, variants, st64, f"learned {name}")
+    sim.close()
+
+
+def test_learned_substep_headline_column_1m(oracle):
+    """The benchmark's learned path at full size: the 1,024,000-particle DP column
+    landed (300 substeps), one learned substep so that the previous-lambda feature
+    comes from a learned start, then one more (20 iterations, the first from the
+    network inside the features pass) against the f32 oracle, its 1-ulp
+    perturbations and the f64 oracle, each given the same network."""
+    scene = scenes.sand_column_1m()
+    sim = make_sim(scene)
+    print("headline column: settling 300 substeps", flush=True)
+    sim.step(300, 20, False)
+    net = _net(5)
+    sim.set_warmstart(net)
+    sim.substep(20, False)
+    st32 = snapshot(sim, oracle)
+    st64 = st32.copy(np.float64)
+    variants = [st32] + [perturbed(st32, k) for k in (1, 2)]
+    mats = mats_record(scene)
+    sim.substep(20, False)
+    for k, st in enumerate(variants + [st64]):
+        print(f"oracle run {k + 1} of {len(variants) + 1}", flush=True)
+        oracle.substep(st, sim.params, mats, 20, False, warm=net)
+    assert np.array_equal(sim.read("PERM"), st32.sortedIdx)
+    compare(sim, variants, st64, "learned 1M DP column after 301 substeps")
     sim.close()
 
 

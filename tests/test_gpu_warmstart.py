@@ -7,7 +7,7 @@
 * a learned substep (fixed iteration count, so that both stop together) matches the
   oracle's learned substep to the bar of the plain parity tests, on DP sand and on
   Herschel-Bulkley goo (whose return map takes the network's bracket hint), and at
-  full size on the benchmark's 1M DP column.
+  full size on the benchmark's 1M DP column and the 1M von Mises poke.
 """
 import numpy as np
 import pytest
@@ -161,6 +161,44 @@ def test_learned_substep_headline_column_1m(oracle):
         oracle.substep(st, sim.params, mats, 20, False, warm=net)
     assert np.array_equal(sim.read("PERM"), st32.sortedIdx)
     compare(sim, variants, st64, "learned 1M DP column after 301 substeps")
+    sim.close()
+
+
+def test_learned_substep_vm_poke_1m(oracle):
+    """The learned start on the 1M von Mises poke at full size: 400 substeps with the
+    sphere moving down at 1 m/s (the previous-lambda feature from learned substeps),
+    then one learned substep with the sphere pressing into the block against the f32
+    oracle, its 1-ulp perturbations and the f64 oracle, each given the same network."""
+    scene = scenes.vm_poke_1m()
+    sim = xpbi.Simulation(scene, device=0)
+    col, dt, I = scene["collider"], scene["dt"], scene["iterations"]
+
+    def place(k):
+        pos = [col["pos"][j] + col["vel"][j] * k * dt for j in range(3)]
+        sim.writeParams(xpbi.params_from_scene(scene, collider=dict(pos=pos, R=col["R"], vel=col["vel"])))
+
+    net = _net(7)
+    sim.set_warmstart(net)
+    steps = 400
+    print(f"von Mises poke 1M, learned start: {steps} substeps with the sphere moving", flush=True)
+    for k in range(steps):
+        place(k)
+        sim.substep(I, False)
+    place(steps)
+    c = np.asarray(col["pos"]) + np.asarray(col["vel"]) * steps * dt
+    inside = int((np.linalg.norm(sim.read("POS")[:, :3] - c, axis=1) < col["R"]).sum())
+    print(f"particles inside the sphere before the checked substep: {inside}", flush=True)
+    assert inside > 100, "the sphere does not press into the block"
+    st32 = snapshot(sim, oracle)
+    st64 = st32.copy(np.float64)
+    variants = [st32] + [perturbed(st32, k) for k in (1, 2)]
+    mats = mats_record(scene)
+    sim.substep(I, False)
+    for k, st in enumerate(variants + [st64]):
+        print(f"oracle run {k + 1} of {len(variants) + 1}", flush=True)
+        oracle.substep(st, sim.params, mats, I, False, warm=net)
+    assert np.array_equal(sim.read("PERM"), st32.sortedIdx)
+    compare(sim, variants, st64, f"learned 1M von Mises poke after {steps} substeps")
     sim.close()
 
 

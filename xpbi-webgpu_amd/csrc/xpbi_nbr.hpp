@@ -338,10 +338,14 @@ __device__ __forceinline__ f3 laneSum(f3 v) { return f3{laneSum<kL>(v.x), laneSu
 // run).  Returns the sum of particle first + threadIdx.x, which the thread then
 // finishes.  Acc provides zero(), laneSum<kL>(), put(lds, slot) and
 // get(lds, slot).  Contains block barriers: every thread of the block calls it.
+// The gathers run at wave priority 1 and the finish at 0: a SIMD then issues
+// first for the waves whose loads keep the address unit busy, and fills the
+// rest with the finishes' VALU work.
 template <u32 kL, class Acc, class Gather>
 __device__ __forceinline__ Acc runPasses(float (*lds)[256], u32 first, u32 pEnd, Gather&& gather) {
   constexpr u32 kPer = 256u / kL;  // particles per pass
   const u32 q = threadIdx.x & (kL - 1u), g = threadIdx.x / kL;
+  __builtin_amdgcn_s_setprio(1);
 #pragma unroll 1
   for (u32 pass = 0; pass < kL; pass++) {
     const u32 base = first + kPer * pass;
@@ -352,6 +356,7 @@ __device__ __forceinline__ Acc runPasses(float (*lds)[256], u32 first, u32 pEnd,
     a.template laneSum<kL>();
     if (q == 0u) a.put(lds, kPer * pass + g);
   }
+  __builtin_amdgcn_s_setprio(0);
   __syncthreads();
   const Acc own = Acc::get(lds, threadIdx.x);
   __syncthreads();  // the next run of a persistent block rewrites the slots
